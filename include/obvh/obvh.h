@@ -1,0 +1,175 @@
+/*
+ * obvh.h -- C ABI of the MI355X (gfx950) OBB engine.
+ *
+ * One shared library, obvh_amd/libobvh.so, exports every symbol below.  The
+ * first group keeps the exact signatures and synchronous semantics of the
+ * reference's GPU entry points, so a plugin that used to compile the CUDA
+ * sources in links this library instead.  The second group is new: device-
+ * resident, stream-ordered entry points for SoA point sets, batches of sets,
+ * the LBVH builder and the transform with a caller-owned workspace.
+ *
+ * All types are plain C with fixed layouts; sizes are checked by
+ * static asserts at the bottom of this file.
+ *
+ * Reference interfaces (paths relative to the reference repository root):
+ *   kr_vec3 / kr_uvec4 / kr_obb3 / kr_mat4   src/common/vecmath.h:136,172,156,158
+ *   kr_bvh_node                              src/common/ads.h:131-142
+ *   kr_error (NULL = success)                src/common/vecmath.h:73,21-24
+ */
+#ifndef OBVH_OBVH_H
+#define OBVH_OBVH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Types (layouts identical to the reference)                              */
+/* ------------------------------------------------------------------------ */
+
+typedef uint32_t u32;
+typedef uint64_t kr_size;
+typedef float kr_scalar;
+typedef const void* kr_error; /* NULL on success, else a static message */
+
+typedef struct { float x, y, z; } kr_vec3;                 /* 12 B */
+typedef struct { u32 x, y, z, w; } kr_uvec4;               /* 16 B, face = x,y,z; w unused */
+typedef struct { kr_vec3 mid, v0, v1, v2, ext; } kr_obb3;  /* 60 B */
+typedef struct { float v[16]; } kr_mat4;                   /* 64 B, column major */
+typedef struct { kr_vec3 min, max; } kr_aabb3;             /* 24 B */
+
+/* 36 B BVH node.  Node array layout: internal nodes [0, N-1), leaves
+ * [N-1, 2N-1).  Leaf: nPrimitives > 0, primitivesOffset = face id.
+ * Internal: nPrimitives == 0, left/right = child node indices.       */
+typedef struct {
+    kr_aabb3 bounds;
+    union {
+        u32 primitivesOffset;
+        struct { u32 left, right; } children;
+    } u;
+    uint16_t nPrimitives;
+    uint8_t axis;
+    uint8_t pad_;
+} kr_bvh_node;
+
+/* ------------------------------------------------------------------------ */
+/* Legacy entry points (drop-in)                                            */
+/* ------------------------------------------------------------------------ */
+
+/* DiTO-14 OBB of a host AoS point set.  Replaces kr_cuda_points_obb
+ * (src/common/algorithm/dito/dito.cuh:12-13, body dito.cu:614-744).
+ * `points` is host memory, borrowed; returns by value; `elapsed_time`
+ * (nullable) receives the device time of the fit in ms.  The result equals
+ * the reference CPU path kr_points_obb (src/common/geometry.cpp:232-252),
+ * i.e. DiTO_14<float> followed by the 0.001 extent floor.  On an internal
+ * error the returned box is all NaN.                                       */
+kr_obb3 kr_cuda_points_obb(const kr_vec3* points, kr_size count, kr_scalar* elapsed_time);
+
+/* AABB-BVH -> OBB-tree transform.  Replaces kr_cuda_bvh_obb_tree
+ * (src/ads/common/cuda/transforms.cuh:52-65, body transforms.cu:966-1059).
+ * All pointers are device pointers owned by the caller.  Writes obbs[] and
+ * transforms[] for all primitive_count + internal_count nodes; never writes
+ * nodes[].  primitive_counts, costs, primitives, ci, ct, obb_cost and
+ * primitive_threshold are accepted and ignored, as in the reference.
+ * Synchronous: returns after the device work has finished.  Returns NULL
+ * on success, else an error string (bad sizes, malformed tree, HIP error). */
+kr_error kr_cuda_bvh_obb_tree(
+    kr_bvh_node* nodes, const u32* parents, const u32* primitive_counts,
+    const kr_scalar* costs, const kr_vec3* vertices, const kr_uvec4* faces,
+    const u32* primitives, kr_obb3* obbs, kr_mat4* transforms,
+    kr_scalar ci, kr_scalar ct, kr_scalar obb_cost,
+    u32 primitive_count, u32 internal_count, u32 primitive_threshold);
+
+/* ------------------------------------------------------------------------ */
+/* Device-resident, stream-ordered entry points (new; no reference          */
+/* counterpart).  `stream` is a hipStream_t passed as void* (NULL = the     */
+/* default stream).  None of these synchronise the host.                    */
+/* ------------------------------------------------------------------------ */
+
+/* Per-fit trace: which points DiTO selected.  Indices into the input set
+ * (slab extremes) or into the candidate point list `sel` (see below).
+ * sel = [min0..min6, max0..max6] for count > 14, else the input order.     */
+typedef struct {
+    u32 slab_min_idx[7];
+    u32 slab_max_idx[7];
+    int32_t pair_k;        /* furthest slab pair, -1 if not reached        */
+    int32_t p2_sel;        /* base-triangle apex, index into sel, or -1    */
+    int32_t q0_sel;        /* max point along the base normal, or -1       */
+    int32_t q1_sel;        /* min point along the base normal, or -1       */
+    int32_t q0_valid, q1_valid;
+    int32_t degenerate;    /* 0 ok, 1 AABB (pair too close), 2 line-aligned, 3 empty */
+    int32_t best;          /* winning candidate 0..20, -1 = initial AABB axes */
+    int32_t result_aabb;   /* 1 if the returned box is the axis-aligned one */
+    int32_t pad_;
+} obvh_dito_trace;       /* 96 B */
+
+/* Workspace bytes needed by obvh_dito_soa for `count` points. */
+size_t obvh_dito_workspace_size(kr_size count);
+
+/* DiTO-14 (+ 0.001 extent floor) of one SoA point set in device memory.
+ * out_obb: device kr_obb3; trace: device obvh_dito_trace or NULL.          */
+kr_error obvh_dito_soa(const float* x, const float* y, const float* z, kr_size count,
+                       kr_obb3* out_obb, obvh_dito_trace* trace,
+                       void* workspace, size_t workspace_bytes, void* stream);
+
+/* Batched DiTO over `nsets` sets stored back to back in SoA arrays;
+ * set s is points [offsets[s], offsets[s+1]).  offsets: device, nsets+1
+ * entries.  tile_points: points per work tile (multiple of 256; 0 = auto).
+ * One persistent launch per pass; the whole call is graph-capturable.     */
+size_t obvh_dito_batch_workspace_size(u32 nsets, kr_size total_points, u32 tile_points);
+kr_error obvh_dito_batch(const float* x, const float* y, const float* z,
+                         const kr_size* offsets, u32 nsets, kr_size total_points,
+                         kr_obb3* out_obbs, obvh_dito_trace* traces,
+                         u32 tile_points, u32 workers,
+                         void* workspace, size_t workspace_bytes, void* stream);
+
+/* Transform with caller workspace, stream ordered.  Same semantics as
+ * kr_cuda_bvh_obb_tree; `kdop_ids` (device, nullable) receives for every
+ * node the 14 selected extremal vertex ids [min0..min6, max0..max6].      */
+size_t obvh_obb_tree_workspace_size(u32 primitive_count, u32 internal_count);
+kr_error obvh_obb_tree_async(
+    const kr_bvh_node* nodes, const u32* parents,
+    const kr_vec3* vertices, const kr_uvec4* faces,
+    kr_obb3* obbs, kr_mat4* transforms, u32* kdop_ids,
+    u32 primitive_count, u32 internal_count,
+    void* workspace, size_t workspace_bytes, void* stream);
+
+/* Host-side tree validation (parents/children consistent, single root at
+ * node 0, every node reachable once).  Inputs are HOST arrays.  Returns
+ * NULL if the tree is well formed.  Used before launching the bottom-up
+ * kernels so that a malformed tree is an error instead of a hang.        */
+kr_error obvh_validate_tree_host(const kr_bvh_node* nodes, const u32* parents,
+                                 u32 primitive_count, u32 internal_count);
+
+/* LBVH builder (30-bit Morton codes, stable sort, Karras radix tree,
+ * bottom-up AABBs/counts/costs), the input side of the transform.
+ * Replaces the build in src/ads/lbvh_gpu/lbvh.cu:1547-1692.
+ * All pointers are device pointers; nodes/parents/primitive_counts/costs
+ * have 2*face_count-1 entries.                                            */
+size_t obvh_lbvh_workspace_size(u32 face_count);
+kr_error obvh_lbvh_build(const kr_vec3* vertices, const kr_uvec4* faces, u32 face_count,
+                         kr_bvh_node* nodes, u32* parents, u32* primitive_counts,
+                         kr_scalar* costs, u32* mortons_sorted,
+                         void* workspace, size_t workspace_bytes, void* stream);
+
+/* Last error string of the calling host thread (NULL if none). */
+const char* obvh_last_error(void);
+
+/* Library version string. */
+const char* obvh_version(void);
+
+#ifdef __cplusplus
+} /* extern "C" */
+
+static_assert(sizeof(kr_vec3) == 12, "kr_vec3 must be 12 bytes");
+static_assert(sizeof(kr_uvec4) == 16, "kr_uvec4 must be 16 bytes");
+static_assert(sizeof(kr_obb3) == 60, "kr_obb3 must be 60 bytes");
+static_assert(sizeof(kr_mat4) == 64, "kr_mat4 must be 64 bytes");
+static_assert(sizeof(kr_bvh_node) == 36, "kr_bvh_node must be 36 bytes");
+static_assert(sizeof(obvh_dito_trace) == 96, "obvh_dito_trace must be 96 bytes");
+#endif
+
+#endif /* OBVH_OBVH_H */

@@ -1,0 +1,278 @@
+// abi.hip -- the extern "C" entry points declared in include/obvh/obvh.h.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <mutex>
+#include <vector>
+
+#include "common.hpp"
+#include "dito.hpp"
+#include "lbvh.hpp"
+#include "transform.hpp"
+
+namespace {
+
+thread_local char g_err[512];
+thread_local const char* g_last = nullptr;
+
+struct DeviceCache {
+    void* pts = nullptr;
+    size_t pts_bytes = 0;
+    void* ws = nullptr;
+    size_t ws_bytes = 0;
+    void* out = nullptr;  // kr_obb3 + error word
+};
+std::mutex g_mu;
+std::vector<DeviceCache> g_cache;
+
+DeviceCache* cache_for_current(int* dev) {
+    if (hipGetDevice(dev) != hipSuccess) return nullptr;
+    if ((int)g_cache.size() <= *dev) g_cache.resize(*dev + 1);
+    return &g_cache[*dev];
+}
+
+hipError_t grow(void** p, size_t* cur, size_t need) {
+    if (*cur >= need && *p) return hipSuccess;
+    if (*p) {
+        hipError_t e = hipFree(*p);
+        if (e != hipSuccess) return e;
+        *p = nullptr;
+        *cur = 0;
+    }
+    size_t n = need < 256 ? 256 : need;
+    hipError_t e = hipMalloc(p, n);
+    if (e == hipSuccess) *cur = n;
+    return e;
+}
+
+kr_error fail(const char* msg) {
+    g_last = msg;
+    return msg;
+}
+
+}  // namespace
+
+void obvh_set_error(const char* msg) { g_last = msg; }
+
+const char* obvh_hip_error(hipError_t e, const char* where) {
+    snprintf(g_err, sizeof(g_err), "HIP error %d (%s) at %s", (int)e, hipGetErrorString(e), where);
+    g_last = g_err;
+    return g_err;
+}
+
+#define TRY(expr)                                                   \
+    do {                                                            \
+        kr_error r__ = (expr);                                      \
+        if (r__) { g_last = (const char*)r__; return r__; }         \
+    } while (0)
+
+extern "C" {
+
+const char* obvh_last_error(void) { return g_last; }
+const char* obvh_version(void) { return "obvh-mi355x 0.1 (gfx950)"; }
+
+// ------------------------------------------------------------ DiTO --
+
+size_t obvh_dito_workspace_size(kr_size count) {
+    obvh::DitoWs ws;
+    uint32_t tile = obvh::dito_auto_tile(1, count);
+    return obvh::dito_carve(&ws, nullptr, 0, 1, count, tile);
+}
+
+kr_error obvh_dito_soa(const float* x, const float* y, const float* z, kr_size count, kr_obb3* out_obb,
+                       obvh_dito_trace* trace, void* workspace, size_t workspace_bytes, void* stream) {
+    if (!out_obb) return fail("obvh_dito_soa: out_obb is NULL");
+    if (count > 0 && (!x || !y || !z)) return fail("obvh_dito_soa: NULL point array");
+    if (count > 0xFFFFFFF0ull) return fail("obvh_dito_soa: more than 2^32 - 16 points");
+    TRY(obvh::dito_launch(x, y, z, nullptr, nullptr, count, 1, count, out_obb, trace, 0, 0, workspace,
+                          workspace_bytes, (hipStream_t)stream));
+    return nullptr;
+}
+
+size_t obvh_dito_batch_workspace_size(u32 nsets, kr_size total_points, u32 tile_points) {
+    obvh::DitoWs ws;
+    uint32_t tile = tile_points ? tile_points : obvh::dito_auto_tile(nsets, total_points);
+    return obvh::dito_carve(&ws, nullptr, 0, nsets, total_points, tile);
+}
+
+kr_error obvh_dito_batch(const float* x, const float* y, const float* z, const kr_size* offsets, u32 nsets,
+                         kr_size total_points, kr_obb3* out_obbs, obvh_dito_trace* traces, u32 tile_points,
+                         u32 workers, void* workspace, size_t workspace_bytes, void* stream) {
+    if (nsets == 0) return nullptr;
+    if (!offsets || !out_obbs) return fail("obvh_dito_batch: NULL offsets or output");
+    if (total_points > 0 && (!x || !y || !z)) return fail("obvh_dito_batch: NULL point array");
+    TRY(obvh::dito_launch(x, y, z, nullptr, (const uint64_t*)offsets, 0, nsets, total_points, out_obbs, traces,
+                          tile_points, workers, workspace, workspace_bytes, (hipStream_t)stream));
+    return nullptr;
+}
+
+// Legacy drop-in: host AoS points in, box by value (dito.cuh:12-13).
+kr_obb3 kr_cuda_points_obb(const kr_vec3* points, kr_size count, kr_scalar* elapsed_time) {
+    kr_obb3 nan_box;
+    float* nb = reinterpret_cast<float*>(&nan_box);
+    for (int i = 0; i < 15; ++i) nb[i] = NAN;
+    if (elapsed_time) *elapsed_time = 0.0f;
+    if (count == 0) {  // DiTO of no points + extent floor
+        kr_obb3 o;
+        o.mid = kr_vec3{0, 0, 0};
+        o.v0 = kr_vec3{1, 0, 0};
+        o.v1 = kr_vec3{0, 1, 0};
+        o.v2 = kr_vec3{0, 0, 1};
+        o.ext = kr_vec3{0.001f, 0.001f, 0.001f};
+        return o;
+    }
+    if (!points || count > 0xFFFFFFF0ull) { fail("kr_cuda_points_obb: bad arguments"); return nan_box; }
+    std::lock_guard<std::mutex> lock(g_mu);
+    int dev = 0;
+    DeviceCache* c = cache_for_current(&dev);
+    if (!c) { fail("kr_cuda_points_obb: no HIP device"); return nan_box; }
+    obvh::DitoWs wsd;
+    const uint32_t tile = obvh::dito_auto_tile(1, count);
+    const size_t wsb = obvh::dito_carve(&wsd, nullptr, 0, 1, count, tile);
+    hipError_t e;
+    if ((e = grow(&c->pts, &c->pts_bytes, count * sizeof(kr_vec3))) != hipSuccess ||
+        (e = grow(&c->ws, &c->ws_bytes, wsb)) != hipSuccess) {
+        obvh_hip_error(e, "kr_cuda_points_obb: hipMalloc");
+        return nan_box;
+    }
+    if (!c->out && (e = hipMalloc(&c->out, 256)) != hipSuccess) {
+        obvh_hip_error(e, "kr_cuda_points_obb: hipMalloc");
+        return nan_box;
+    }
+    if ((e = hipMemcpy(c->pts, points, count * sizeof(kr_vec3), hipMemcpyHostToDevice)) != hipSuccess) {
+        obvh_hip_error(e, "kr_cuda_points_obb: hipMemcpy H2D");
+        return nan_box;
+    }
+    hipEvent_t ev0, ev1;
+    hipEventCreate(&ev0);
+    hipEventCreate(&ev1);
+    hipEventRecord(ev0, nullptr);
+    kr_error r = obvh::dito_launch(nullptr, nullptr, nullptr, (const float*)c->pts, nullptr, count, 1, count,
+                                   (kr_obb3*)c->out, nullptr, tile, 0, c->ws, c->ws_bytes, nullptr);
+    hipEventRecord(ev1, nullptr);
+    kr_obb3 o = nan_box;
+    if (!r) {
+        if ((e = hipMemcpy(&o, c->out, sizeof(kr_obb3), hipMemcpyDeviceToHost)) != hipSuccess) {
+            obvh_hip_error(e, "kr_cuda_points_obb: hipMemcpy D2H");
+            o = nan_box;
+        } else if (elapsed_time) {
+            float ms = 0.0f;
+            hipEventElapsedTime(&ms, ev0, ev1);
+            *elapsed_time = ms;
+        }
+    } else {
+        g_last = (const char*)r;
+    }
+    hipEventDestroy(ev0);
+    hipEventDestroy(ev1);
+    return o;
+}
+
+// ------------------------------------------------------- transform --
+
+size_t obvh_obb_tree_workspace_size(u32 primitive_count, u32 internal_count) {
+    (void)internal_count;
+    obvh::TreeWs ws;
+    return obvh::tree_carve(&ws, nullptr, 0, primitive_count);
+}
+
+kr_error obvh_obb_tree_async(const kr_bvh_node* nodes, const u32* parents, const kr_vec3* vertices,
+                             const kr_uvec4* faces, kr_obb3* obbs, kr_mat4* transforms, u32* kdop_ids,
+                             u32 primitive_count, u32 internal_count, void* workspace, size_t workspace_bytes,
+                             void* stream) {
+    if (primitive_count == 0) return nullptr;
+    if (internal_count != primitive_count - 1)
+        return fail("obvh_obb_tree: internal_count must be primitive_count - 1");
+    if (primitive_count > 0x7FFFFFFFu) return fail("obvh_obb_tree: too many primitives");
+    if (!nodes || !parents || !vertices || !faces || !obbs || !transforms)
+        return fail("obvh_obb_tree: NULL array");
+    TRY(obvh::tree_launch(nodes, parents, vertices, faces, obbs, transforms, kdop_ids, primitive_count, workspace,
+                          workspace_bytes, (hipStream_t)stream));
+    return nullptr;
+}
+
+// Legacy drop-in (transforms.cuh:52-65): synchronous, scratch owned here.
+kr_error kr_cuda_bvh_obb_tree(kr_bvh_node* nodes, const u32* parents, const u32* primitive_counts,
+                              const kr_scalar* costs, const kr_vec3* vertices, const kr_uvec4* faces,
+                              const u32* primitives, kr_obb3* obbs, kr_mat4* transforms, kr_scalar ci,
+                              kr_scalar ct, kr_scalar obb_cost, u32 primitive_count, u32 internal_count,
+                              u32 primitive_threshold) {
+    (void)primitive_counts; (void)costs; (void)primitives; (void)ci; (void)ct; (void)obb_cost;
+    (void)primitive_threshold;
+    if (primitive_count == 0) return nullptr;
+    if (internal_count != primitive_count - 1)
+        return fail("kr_cuda_bvh_obb_tree: internal_count must be primitive_count - 1");
+    std::lock_guard<std::mutex> lock(g_mu);
+    int dev = 0;
+    DeviceCache* c = cache_for_current(&dev);
+    if (!c) return fail("kr_cuda_bvh_obb_tree: no HIP device");
+    obvh::TreeWs wsd;
+    size_t need = obvh::tree_carve(&wsd, nullptr, 0, primitive_count);
+    OBVH_HIP_TRY(grow(&c->ws, &c->ws_bytes, need));
+    TRY(obvh::tree_launch(nodes, parents, vertices, faces, obbs, transforms, nullptr, primitive_count, c->ws,
+                          c->ws_bytes, nullptr));
+    obvh::TreeWs ws;
+    obvh::tree_carve(&ws, c->ws, c->ws_bytes, primitive_count);
+    uint32_t err = 0;
+    OBVH_HIP_TRY(hipMemcpy(&err, ws.error, sizeof(err), hipMemcpyDeviceToHost));  // synchronises
+    if (err & 2u) return fail("kr_cuda_bvh_obb_tree: parent/child links disagree");
+    if (err & 1u) return fail("kr_cuda_bvh_obb_tree: parent index out of range or cycle");
+    if (err & 4u) return fail("kr_cuda_bvh_obb_tree: leaf face id out of range");
+    if (err & 8u) return fail("kr_cuda_bvh_obb_tree: internal node never completed (malformed tree)");
+    return nullptr;
+}
+
+kr_error obvh_validate_tree_host(const kr_bvh_node* nodes, const u32* parents, u32 primitive_count,
+                                 u32 internal_count) {
+    if (primitive_count == 0) return nullptr;
+    if (internal_count != primitive_count - 1) return fail("tree: internal_count != primitive_count - 1");
+    const uint32_t I = internal_count, total = I + primitive_count;
+    if (parents[0] != 0xFFFFFFFFu) return fail("tree: root (node 0) has a parent");
+    for (uint32_t i = 0; i < I; ++i) {
+        if (nodes[i].nPrimitives != 0) return fail("tree: internal node with primitives");
+        uint32_t l = nodes[i].u.children.left, r = nodes[i].u.children.right;
+        if (l >= total || r >= total || l == 0 || r == 0 || l == r) return fail("tree: bad child index");
+        if (parents[l] != i || parents[r] != i) return fail("tree: parent/child links disagree");
+    }
+    for (uint32_t i = I; i < total; ++i)
+        if (nodes[i].nPrimitives == 0) return fail("tree: leaf without primitives");
+    // every node reachable exactly once from the root
+    std::vector<uint32_t> stack;
+    std::vector<uint8_t> seen(total, 0);
+    stack.push_back(0);
+    uint32_t count = 0;
+    while (!stack.empty()) {
+        uint32_t v = stack.back();
+        stack.pop_back();
+        if (seen[v]) return fail("tree: node reached twice");
+        seen[v] = 1;
+        ++count;
+        if (v < I) {
+            stack.push_back(nodes[v].u.children.left);
+            stack.push_back(nodes[v].u.children.right);
+        }
+    }
+    if (count != total) return fail("tree: unreachable nodes");
+    return nullptr;
+}
+
+// ------------------------------------------------------------ LBVH --
+
+size_t obvh_lbvh_workspace_size(u32 face_count) {
+    obvh::LbvhWs ws;
+    return obvh::lbvh_carve(&ws, nullptr, 0, face_count);
+}
+
+kr_error obvh_lbvh_build(const kr_vec3* vertices, const kr_uvec4* faces, u32 face_count, kr_bvh_node* nodes,
+                         u32* parents, u32* primitive_counts, kr_scalar* costs, u32* mortons_sorted,
+                         void* workspace, size_t workspace_bytes, void* stream) {
+    if (!vertices || !faces || !nodes || !parents || !primitive_counts || !costs)
+        return fail("obvh_lbvh_build: NULL array");
+    if (face_count > 0x7FFFFFFFu) return fail("obvh_lbvh_build: too many faces");
+    TRY(obvh::lbvh_launch(vertices, faces, face_count, nodes, parents, primitive_counts, costs, mortons_sorted,
+                          workspace, workspace_bytes, (hipStream_t)stream));
+    return nullptr;
+}
+
+}  // extern "C"

@@ -1,0 +1,656 @@
+// dito.hip -- DiTO-14 oriented bounding boxes of point sets on gfx950.
+//
+// Method: Larsson & Kallberg DiTO-14 exactly as the reference CPU code
+// computes it (src/common/algorithm/dito/dito.cpp:478-538), plus the 0.001
+// extent floor of kr_points_obb (src/common/geometry.cpp:249).  The GPU
+// driver of the reference (dito.cu:614-744: four synchronised launches,
+// float CAS atomics per point, last-writer index selection) is replaced by:
+//
+//  setup     one workgroup scans the set sizes into a tile table (a set is
+//            cut into tiles of `tile` points) and writes the result of empty
+//            sets; a grid kernel fills tile -> set.
+//  pass 1    persistent workgroups (4 x wave64) stream tiles.  Lane l of a
+//            wave reads points chunk + j*64 + l, j < 8 (coalesced SoA or AoS),
+//            forms the 7 slab projections, and keeps per slab side the running
+//            extreme and the chunk where it last strictly improved.  The lane
+//            holding a wave's winner re-reads its 8 points to recover the
+//            exact first index, so index bookkeeping costs one compare per
+//            8 points.  (value, index) pairs reduce lexicographically (ties
+//            keep the lower index): order free, and equal to the reference's
+//            sequential strict-compare scan.  Output: 14 pairs per tile.
+//  cand      one wave per set reduces its tile partials, then one lane runs
+//            the candidate search (base triangle + ditetrahedron, up to 21
+//            axis triples) over a 14-entry point array held in registers.
+//  pass 2    the same tiles project every point on the 3 chosen axes (fused
+//            min/max of 3 dot products), 6 floats per tile.
+//  final     one wave per set reduces and finalises the box.
+//
+// No atomics or cross-workgroup fences: kernel boundaries order the phases.
+// Each pass reads 12 B per point, 24 B per point per fit.
+#include <float.h>
+
+#include "common.hpp"
+#include "dito.hpp"
+
+namespace obvh {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / kWave;
+constexpr int kG = 8;               // points per lane per chunk
+constexpr int kChunk = kWave * kG;  // points per wave per chunk
+constexpr uint32_t kNoIdx = 0xFFFFFFFFu;
+
+struct PtsSoA {
+    const float* x;
+    const float* y;
+    const float* z;
+    __device__ __forceinline__ f3 load(uint64_t i) const { return f3{x[i], y[i], z[i]}; }
+};
+struct PtsAoS {
+    const float* xyz;
+    __device__ __forceinline__ f3 load(uint64_t i) const {
+        return f3{xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
+    }
+};
+
+__device__ __forceinline__ uint64_t set_off(const uint64_t* offs, uint64_t single, uint32_t s) {
+    return offs ? offs[s] : (s ? single : 0);
+}
+
+__device__ __forceinline__ float slab_k(f3 p, int k) {
+    float pr[7];
+    slab7(p, pr);
+    float v = pr[0];
+#pragma unroll
+    for (int i = 1; i < 7; ++i)
+        if (k == i) v = pr[i];
+    return v;
+}
+
+// ---------------------------------------------------------------- setup --
+
+__global__ __launch_bounds__(1024) void dito_setup(const uint64_t* offs, uint64_t single_count,
+                                                   uint32_t nsets, uint32_t tile, DitoWs ws,
+                                                   kr_obb3* out, obvh_dito_trace* trace) {
+    __shared__ uint32_t carry;
+    __shared__ uint32_t wsum[1024 / kWave];
+    if (threadIdx.x == 0) carry = 0;
+    if (threadIdx.x == 0 && offs == nullptr) {
+        ws.offsets_single[0] = 0;
+        ws.offsets_single[1] = single_count;
+    }
+    __syncthreads();
+    for (uint32_t base = 0; base < nsets; base += blockDim.x) {
+        const uint32_t s = base + threadIdx.x;
+        uint32_t nt = 0;
+        if (s < nsets) {
+            const uint64_t n = set_off(offs, single_count, s + 1) - set_off(offs, single_count, s);
+            nt = (uint32_t)((n + tile - 1) / tile);
+            if (n == 0) {
+                // DiTO_14 of an empty set (dito.cpp:498-502) + extent floor.
+                kr_obb3 o;
+                o.mid = kr_vec3{0, 0, 0};
+                o.v0 = kr_vec3{1, 0, 0};
+                o.v1 = kr_vec3{0, 1, 0};
+                o.v2 = kr_vec3{0, 0, 1};
+                o.ext = kr_vec3{0.001f, 0.001f, 0.001f};
+                out[s] = o;
+                obvh_dito_trace t = {};
+                t.pair_k = t.p2_sel = t.q0_sel = t.q1_sel = -1;
+                t.best = -1;
+                t.degenerate = 3;
+                t.result_aabb = 1;
+                trace[s] = t;
+            }
+        }
+        uint32_t v = nt;
+        const int lane = lane_id();
+#pragma unroll
+        for (int m = 1; m < kWave; m <<= 1) {
+            const uint32_t o = (uint32_t)__shfl_up((int)v, m, kWave);
+            if (lane >= m) v += o;
+        }
+        if (lane == kWave - 1) wsum[wave_id()] = v;
+        __syncthreads();
+        uint32_t wpre = 0;
+        for (int w = 0; w < wave_id(); ++w) wpre += wsum[w];
+        const uint32_t incl = carry + wpre + v;
+        if (s < nsets) ws.first_tile[s] = incl - nt;
+        __syncthreads();
+        if (threadIdx.x == blockDim.x - 1) carry = incl;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) ws.first_tile[nsets] = carry;
+}
+
+__global__ __launch_bounds__(256) void dito_tile_table(uint32_t nsets, DitoWs ws) {
+    const uint32_t T = ws.first_tile[nsets];
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < T; t += gridDim.x * blockDim.x) {
+        uint32_t lo = 0, hi = nsets;  // first_tile[lo] <= t < first_tile[hi]
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (ws.first_tile[mid] <= t) lo = mid; else hi = mid;
+        }
+        ws.tile_set[t] = lo;  // last set starting at or before t: skips empty sets
+    }
+}
+
+// --------------------------------------------------------------- pass 1 --
+
+template <class P>
+__global__ __launch_bounds__(kBlock) void dito_pass1(P pts, const uint64_t* offs, uint32_t nsets,
+                                                     uint32_t tile, DitoWs ws) {
+    __shared__ float sh_v[kWaves][14];
+    __shared__ uint32_t sh_i[kWaves][14];
+    const uint32_t T = ws.first_tile[nsets];
+    const int lane = lane_id(), wave = wave_id();
+    const float NaN = __builtin_nanf("");
+
+    for (uint32_t t = blockIdx.x; t < T; t += gridDim.x) {
+        const uint32_t s = ws.tile_set[t];
+        const uint64_t sbeg = offs[s], send = offs[s + 1];
+        const uint32_t ft = ws.first_tile[s];
+        const uint64_t base = sbeg + (uint64_t)(t - ft) * tile;
+        const uint64_t end = base + tile < send ? base + tile : send;
+        const uint32_t nchunks = (uint32_t)((end - base + kChunk - 1) / kChunk);
+
+        float rmn[7], rmx[7];
+        uint32_t cmn[7], cmx[7];
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            rmn[k] = NaN;  // NaN never compares: the first real group wins
+            rmx[k] = NaN;
+            cmn[k] = cmx[k] = kNoIdx;
+        }
+        for (uint32_t c = wave; c < nchunks; c += kWaves) {
+            const uint64_t cb = base + (uint64_t)c * kChunk + lane;
+            const bool full = base + (uint64_t)(c + 1) * kChunk <= end;
+            f3 p[kG];
+#pragma unroll
+            for (int j = 0; j < kG; ++j) {
+                const uint64_t idx = cb + (uint64_t)j * kWave;
+                p[j] = (full || idx < end) ? pts.load(idx) : f3{NaN, NaN, NaN};
+            }
+            float gmn[7], gmx[7];
+            slab7(p[0], gmn);
+#pragma unroll
+            for (int k = 0; k < 7; ++k) gmx[k] = gmn[k];
+#pragma unroll
+            for (int j = 1; j < kG; ++j) {
+                float pr[7];
+                slab7(p[j], pr);
+#pragma unroll
+                for (int k = 0; k < 7; ++k) {
+                    gmn[k] = fminf(gmn[k], pr[k]);  // NaN placeholders are ignored
+                    gmx[k] = fmaxf(gmx[k], pr[k]);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 7; ++k) {
+                if (gmn[k] < rmn[k] || (cmn[k] == kNoIdx && gmn[k] == gmn[k])) { rmn[k] = gmn[k]; cmn[k] = c; }
+                if (gmx[k] > rmx[k] || (cmx[k] == kNoIdx && gmx[k] == gmx[k])) { rmx[k] = gmx[k]; cmx[k] = c; }
+            }
+        }
+        // Per slab side: wave winner (value, chunk), then its exact first index.
+        for (int side = 0; side < 14; ++side) {
+            const int k = side < 7 ? side : side - 7;
+            const bool isMin = side < 7;
+            float v = rmn[0];
+            uint32_t ch = cmn[0];
+#pragma unroll
+            for (int q = 0; q < 7; ++q) {
+                if (k == q) { v = isMin ? rmn[q] : rmx[q]; ch = isMin ? cmn[q] : cmx[q]; }
+            }
+            float wv = ch == kNoIdx ? (isMin ? FLT_MAX * 2.0f : -FLT_MAX * 2.0f) : v;
+            uint32_t wc = ch;
+            if (isMin) wave_pair<true>(wv, wc); else wave_pair<false>(wv, wc);
+            uint32_t myidx = kNoIdx;
+            float myval = wv;
+            if (wc != kNoIdx && ch == wc && v == wv) {
+                const uint64_t cb = base + (uint64_t)wc * kChunk + lane;
+                for (int j = 0; j < kG; ++j) {
+                    const uint64_t idx = cb + (uint64_t)j * kWave;
+                    if (idx >= end) break;
+                    const float pv = slab_k(pts.load(idx), k);
+                    if (pv == wv) { myidx = (uint32_t)(idx - sbeg); myval = pv; break; }
+                }
+            }
+#pragma unroll
+            for (int m = 1; m < kWave; m <<= 1) {  // lowest index among tied lanes
+                const uint32_t oi = (uint32_t)__shfl_xor((int)myidx, m, kWave);
+                const float ov = __shfl_xor(myval, m, kWave);
+                if (oi < myidx) { myidx = oi; myval = ov; }
+            }
+            if (lane == 0) {
+                sh_v[wave][side] = myval;
+                sh_i[wave][side] = myidx;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < 14) {
+            const int side = threadIdx.x;
+            float v = sh_v[0][side];
+            uint32_t i = sh_i[0][side];
+            for (int w = 1; w < kWaves; ++w) {
+                const float ov = sh_v[w][side];
+                const uint32_t oi = sh_i[w][side];
+                if (oi == kNoIdx) continue;
+                const bool better = (i == kNoIdx) ||
+                                    (side < 7 ? pair_better<true>(v, i, ov, oi) : pair_better<false>(v, i, ov, oi));
+                if (better) { v = ov; i = oi; }
+            }
+            ws.part1_v[(size_t)t * 14 + side] = v;
+            ws.part1_i[(size_t)t * 14 + side] = i;
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------ candidate search (1 lane) --
+//
+// dito.cpp:347-418 over a 14-entry point list `sel`.  Sets of n <= 14 points
+// use their n points followed by copies of sel[0]: a copy never wins a strict
+// comparison against the first occurrence, so every result equals the
+// n-point search.  Candidate ids: 0..2 base triangle, 3..11 upper tetra,
+// 12..20 lower tetra.
+
+struct Search {
+    float best;
+    f3 b0, b1, b2;
+    int best_id;
+};
+
+__device__ __forceinline__ void ext_projs(const f3* sel, f3 d, float& mn, float& mx) {
+    float p = dot3(sel[0], d);
+    mn = p;
+    mx = p;
+#pragma unroll
+    for (int i = 1; i < 14; ++i) {
+        p = dot3(sel[i], d);
+        if (p < mn) mn = p;  // dito.cpp:198-202
+        if (p > mx) mx = p;
+    }
+}
+
+__device__ __forceinline__ void eval_triangle(const f3* sel, Search& s, int id0, f3 n, f3 e0, f3 e1, f3 e2) {
+    float nmin, nmax;
+    ext_projs(sel, n, nmin, nmax);
+    const float leny = nmax - nmin;
+    const f3 e[3] = {e0, e1, e2};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const f3 m = dm::cross(e[k], n);
+        float emin, emax, mmin, mmax;
+        ext_projs(sel, e[k], emin, emax);
+        ext_projs(sel, m, mmin, mmax);
+        const float q = quality(f3{emax - emin, leny, mmax - mmin});
+        if (q < s.best) {
+            s.best = q;
+            s.b0 = e[k];
+            s.b1 = n;
+            s.b2 = m;
+            s.best_id = id0 + k;
+        }
+    }
+}
+
+__device__ void dito_candidate_search(const float mn[7], const float mx[7], const f3 minv[7], const f3 maxv[7],
+                                      const f3 sel[14], SetState& st, obvh_dito_trace& tr) {
+    const float eps = 0.000001f;
+    const f3 almid = f3{(mn[0] + mx[0]) * 0.5f, (mn[1] + mx[1]) * 0.5f, (mn[2] + mx[2]) * 0.5f};
+    const f3 allen = f3{mx[0] - mn[0], mx[1] - mn[1], mx[2] - mn[2]};
+    st.almid[0] = almid.x; st.almid[1] = almid.y; st.almid[2] = almid.z;
+    st.allen[0] = allen.x; st.allen[1] = allen.y; st.allen[2] = allen.z;
+    st.alval = quality(allen);
+
+    Search s;
+    s.best = st.alval;
+    s.b0 = f3{1, 0, 0};
+    s.b1 = f3{0, 1, 0};
+    s.b2 = f3{0, 0, 1};
+    s.best_id = -1;
+
+    int kstar = 0;
+    float dbest = dm::sqdist(maxv[0], minv[0]);
+#pragma unroll
+    for (int k = 1; k < 7; ++k) {
+        const float d = dm::sqdist(maxv[k], minv[k]);
+        if (d > dbest) { dbest = d; kstar = k; }
+    }
+    f3 p0 = minv[0], p1 = maxv[0];
+#pragma unroll
+    for (int k = 1; k < 7; ++k)
+        if (kstar == k) { p0 = minv[k]; p1 = maxv[k]; }
+    tr.pair_k = kstar;
+    if (dm::sqdist(p0, p1) < eps) {  // degenerate case 1 -> AABB
+        st.mode = 1;
+        tr.degenerate = 1;
+        return;
+    }
+    const f3 e0 = dm::normalize(sub3(p0, p1));
+    int ip2 = 0;
+    f3 p2 = sel[0];
+    float dmax;
+    {
+        f3 u0 = sub3(sel[0], p0);
+        float t = dot3(e0, u0);
+        dmax = len2(u0) - t * t / len2(e0);
+#pragma unroll
+        for (int i = 1; i < 14; ++i) {
+            u0 = sub3(sel[i], p0);
+            t = dot3(e0, u0);
+            const float d = len2(u0) - t * t / len2(e0);
+            if (d > dmax) { dmax = d; ip2 = i; p2 = sel[i]; }
+        }
+    }
+    tr.p2_sel = ip2;
+    if (dmax < eps) {  // degenerate case 2 -> line-aligned box (dito.cpp:451-476)
+        const f3 u = e0;
+        f3 r = u;
+        if (fabsf(u.x) > fabsf(u.y) && fabsf(u.x) > fabsf(u.z)) r.x = 0;
+        else if (fabsf(u.y) > fabsf(u.z)) r.y = 0;
+        else r.z = 0;
+        if (len2(r) < eps) r = f3{1, 1, 1};
+        const f3 v = dm::normalize(dm::cross(u, r));
+        const f3 w = dm::normalize(dm::cross(u, v));
+        st.ax[0] = u.x; st.ax[1] = u.y; st.ax[2] = u.z;
+        st.ax[3] = v.x; st.ax[4] = v.y; st.ax[5] = v.z;
+        st.ax[6] = w.x; st.ax[7] = w.y; st.ax[8] = w.z;
+        st.mode = 2;
+        tr.degenerate = 2;
+        return;
+    }
+    const f3 e1 = dm::normalize(sub3(p1, p2));
+    const f3 e2 = dm::normalize(sub3(p2, p0));
+    const f3 n = dm::normalize(dm::cross(e1, e0));
+    eval_triangle(sel, s, 0, n, e0, e1, e2);
+
+    float pr = dot3(sel[0], n);
+    float lo = pr, hi = pr;
+    int ilo = 0, ihi = 0;
+    f3 q1 = sel[0], q0 = sel[0];
+#pragma unroll
+    for (int i = 1; i < 14; ++i) {
+        pr = dot3(sel[i], n);
+        if (pr < lo) { lo = pr; ilo = i; q1 = sel[i]; }
+        if (pr > hi) { hi = pr; ihi = i; q0 = sel[i]; }
+    }
+    const float tri = dot3(p0, n);
+    const int q0v = (hi - eps > tri) ? 1 : 0;
+    const int q1v = (lo + eps < tri) ? 1 : 0;
+    tr.q0_sel = ihi;
+    tr.q1_sel = ilo;
+    tr.q0_valid = q0v;
+    tr.q1_valid = q1v;
+    if (q0v) {
+        const f3 f0 = dm::normalize(sub3(q0, p0));
+        const f3 f1 = dm::normalize(sub3(q0, p1));
+        const f3 f2 = dm::normalize(sub3(q0, p2));
+        eval_triangle(sel, s, 3, dm::normalize(dm::cross(f1, e0)), e0, f1, f0);
+        eval_triangle(sel, s, 6, dm::normalize(dm::cross(f2, e1)), e1, f2, f1);
+        eval_triangle(sel, s, 9, dm::normalize(dm::cross(f0, e2)), e2, f0, f2);
+    }
+    if (q1v) {
+        const f3 g0 = dm::normalize(sub3(q1, p0));
+        const f3 g1 = dm::normalize(sub3(q1, p1));
+        const f3 g2 = dm::normalize(sub3(q1, p2));
+        eval_triangle(sel, s, 12, dm::normalize(dm::cross(g1, e0)), e0, g1, g0);
+        eval_triangle(sel, s, 15, dm::normalize(dm::cross(g2, e1)), e1, g2, g1);
+        eval_triangle(sel, s, 18, dm::normalize(dm::cross(g0, e2)), e2, g0, g2);
+    }
+    st.ax[0] = s.b0.x; st.ax[1] = s.b0.y; st.ax[2] = s.b0.z;
+    st.ax[3] = s.b1.x; st.ax[4] = s.b1.y; st.ax[5] = s.b1.z;
+    st.ax[6] = s.b2.x; st.ax[7] = s.b2.y; st.ax[8] = s.b2.z;
+    st.mode = 0;
+    tr.degenerate = 0;
+    tr.best = s.best_id;
+}
+
+// One wave per set: reduce the tile partials, then lane 0 searches.
+template <class P>
+__global__ __launch_bounds__(kBlock) void dito_cand(P pts, const uint64_t* offs, uint32_t nsets, DitoWs ws,
+                                                    obvh_dito_trace* trace) {
+    const uint32_t s = blockIdx.x * kWaves + wave_id();
+    if (s >= nsets) return;
+    const uint64_t sbeg = offs[s], send = offs[s + 1];
+    if (send == sbeg) return;
+    const uint32_t ft = ws.first_tile[s], nt = ws.first_tile[s + 1] - ft;
+    const int lane = lane_id();
+    float v[14];
+    uint32_t ix[14];
+#pragma unroll
+    for (int side = 0; side < 14; ++side) {
+        v[side] = side < 7 ? FLT_MAX * 2.0f : -FLT_MAX * 2.0f;
+        ix[side] = kNoIdx;
+    }
+    for (uint32_t j = lane; j < nt; j += kWave) {
+#pragma unroll
+        for (int side = 0; side < 14; ++side) {
+            const float ov = ws.part1_v[(size_t)(ft + j) * 14 + side];
+            const uint32_t oi = ws.part1_i[(size_t)(ft + j) * 14 + side];
+            const bool better = (ix[side] == kNoIdx) ||
+                                (side < 7 ? pair_better<true>(v[side], ix[side], ov, oi)
+                                          : pair_better<false>(v[side], ix[side], ov, oi));
+            if (better) { v[side] = ov; ix[side] = oi; }
+        }
+    }
+#pragma unroll
+    for (int side = 0; side < 14; ++side) {
+        if (side < 7) wave_pair<true>(v[side], ix[side]); else wave_pair<false>(v[side], ix[side]);
+    }
+    if (lane != 0) return;
+    const uint64_t n = send - sbeg;
+#pragma unroll
+    for (int side = 0; side < 14; ++side)
+        if ((uint64_t)ix[side] >= n) ix[side] = 0;  // only reachable with NaN-only input: never index past the set
+    obvh_dito_trace tr = {};
+    tr.pair_k = tr.p2_sel = tr.q0_sel = tr.q1_sel = -1;
+    tr.best = -1;
+    float mn[7], mx[7];
+    f3 minv[7], maxv[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        mn[k] = v[k];
+        mx[k] = v[7 + k];
+        tr.slab_min_idx[k] = ix[k];
+        tr.slab_max_idx[k] = ix[7 + k];
+        minv[k] = pts.load(sbeg + ix[k]);
+        maxv[k] = pts.load(sbeg + ix[7 + k]);
+    }
+    f3 sel[14];
+    if (n > 14) {
+#pragma unroll
+        for (int k = 0; k < 7; ++k) { sel[k] = minv[k]; sel[7 + k] = maxv[k]; }
+    } else {
+        sel[0] = pts.load(sbeg);
+#pragma unroll
+        for (int i = 1; i < 14; ++i) sel[i] = (uint64_t)i < n ? pts.load(sbeg + i) : sel[0];
+    }
+    SetState st = {};
+    dito_candidate_search(mn, mx, minv, maxv, sel, st, tr);
+    ws.state[s] = st;
+    trace[s] = tr;
+}
+
+// --------------------------------------------------------------- pass 2 --
+
+template <class P>
+__global__ __launch_bounds__(kBlock) void dito_pass2(P pts, const uint64_t* offs, uint32_t nsets,
+                                                     uint32_t tile, DitoWs ws) {
+    __shared__ float sh[kWaves][6];
+    const uint32_t T = ws.first_tile[nsets];
+    const int lane = lane_id(), wave = wave_id();
+    const float INF = FLT_MAX * 2.0f, NaN = __builtin_nanf("");
+
+    for (uint32_t t = blockIdx.x; t < T; t += gridDim.x) {
+        const uint32_t s = ws.tile_set[t];
+        const uint64_t sbeg = offs[s], send = offs[s + 1];
+        const uint32_t ft = ws.first_tile[s];
+        const uint64_t base = sbeg + (uint64_t)(t - ft) * tile;
+        const uint64_t end = base + tile < send ? base + tile : send;
+        const SetState* st = &ws.state[s];
+        float r[6] = {INF, INF, INF, -INF, -INF, -INF};
+        if (st->mode != 1) {
+            const f3 b0 = f3{st->ax[0], st->ax[1], st->ax[2]};
+            const f3 b1 = f3{st->ax[3], st->ax[4], st->ax[5]};
+            const f3 b2 = f3{st->ax[6], st->ax[7], st->ax[8]};
+            const uint32_t nchunks = (uint32_t)((end - base + kChunk - 1) / kChunk);
+            for (uint32_t c = wave; c < nchunks; c += kWaves) {
+                const uint64_t cb = base + (uint64_t)c * kChunk + lane;
+                const bool full = base + (uint64_t)(c + 1) * kChunk <= end;
+                f3 p[kG];
+#pragma unroll
+                for (int j = 0; j < kG; ++j) {
+                    const uint64_t idx = cb + (uint64_t)j * kWave;
+                    p[j] = (full || idx < end) ? pts.load(idx) : f3{NaN, NaN, NaN};
+                }
+#pragma unroll
+                for (int j = 0; j < kG; ++j) {
+                    const float d0 = dot3(p[j], b0), d1 = dot3(p[j], b1), d2 = dot3(p[j], b2);
+                    r[0] = fminf(r[0], d0); r[3] = fmaxf(r[3], d0);
+                    r[1] = fminf(r[1], d1); r[4] = fmaxf(r[4], d1);
+                    r[2] = fminf(r[2], d2); r[5] = fmaxf(r[5], d2);
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { r[k] = wave_min(r[k]); r[3 + k] = wave_max(r[3 + k]); }
+        if (lane == 0) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) sh[wave][k] = r[k];
+        }
+        __syncthreads();
+        if (threadIdx.x < 6) {
+            const int k = threadIdx.x;
+            float a = sh[0][k];
+            for (int w = 1; w < kWaves; ++w) a = k < 3 ? fminf(a, sh[w][k]) : fmaxf(a, sh[w][k]);
+            ws.part2[(size_t)t * 6 + k] = a;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- final --
+
+__global__ __launch_bounds__(kBlock) void dito_final(const uint64_t* offs, uint32_t nsets, DitoWs ws,
+                                                     kr_obb3* out, obvh_dito_trace* trace) {
+    const uint32_t s = blockIdx.x * kWaves + wave_id();
+    if (s >= nsets) return;
+    if (offs[s + 1] == offs[s]) return;
+    const uint32_t ft = ws.first_tile[s], nt = ws.first_tile[s + 1] - ft;
+    const float INF = FLT_MAX * 2.0f;
+    float a[6] = {INF, INF, INF, -INF, -INF, -INF};
+    for (uint32_t j = lane_id(); j < nt; j += kWave) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const float o = ws.part2[(size_t)(ft + j) * 6 + k];
+            a[k] = k < 3 ? fminf(a[k], o) : fmaxf(a[k], o);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { a[k] = wave_min(a[k]); a[3 + k] = wave_max(a[3 + k]); }
+    if (lane_id() != 0) return;
+    const SetState st = ws.state[s];
+    const f3 b0 = f3{st.ax[0], st.ax[1], st.ax[2]};
+    const f3 b1 = f3{st.ax[3], st.ax[4], st.ax[5]};
+    const f3 b2 = f3{st.ax[6], st.ax[7], st.ax[8]};
+    const f3 bmn = f3{a[0], a[1], a[2]}, bmx = f3{a[3], a[4], a[5]};
+    const f3 blen = sub3(bmx, bmn);
+    kr_obb3 o;
+    int result_aabb;
+    if (st.mode == 2 || (st.mode == 0 && quality(blen) < st.alval)) {
+        // finalizeOBB, dito.cpp:428-440
+        const f3 ext = dm::scale(0.5f, blen);
+        const f3 q = dm::scale(0.5f, add3(bmn, bmx));
+        f3 mid = dm::scale(q.x, b0);
+        mid = add3(mid, dm::scale(q.y, b1));
+        mid = add3(mid, dm::scale(q.z, b2));
+        o.mid = kr_vec3{mid.x, mid.y, mid.z};
+        o.v0 = kr_vec3{b0.x, b0.y, b0.z};
+        o.v1 = kr_vec3{b1.x, b1.y, b1.z};
+        o.v2 = kr_vec3{b2.x, b2.y, b2.z};
+        o.ext = kr_vec3{ext.x, ext.y, ext.z};
+        result_aabb = 0;
+    } else {  // finalizeAxisAlignedOBB, dito.cpp:442-449
+        const f3 e = dm::scale(0.5f, f3{st.allen[0], st.allen[1], st.allen[2]});
+        o.mid = kr_vec3{st.almid[0], st.almid[1], st.almid[2]};
+        o.v0 = kr_vec3{1, 0, 0};
+        o.v1 = kr_vec3{0, 1, 0};
+        o.v2 = kr_vec3{0, 0, 1};
+        o.ext = kr_vec3{e.x, e.y, e.z};
+        result_aabb = 1;
+    }
+    // kr_points_obb extent floor (geometry.cpp:249)
+    o.ext.x = kmax(o.ext.x, 0.001f);
+    o.ext.y = kmax(o.ext.y, 0.001f);
+    o.ext.z = kmax(o.ext.z, 0.001f);
+    out[s] = o;
+    trace[s].result_aabb = result_aabb;
+}
+
+// ----------------------------------------------------------------- host --
+
+size_t dito_max_tiles(uint32_t nsets, uint64_t total, uint32_t tile) {
+    return (size_t)(total / tile) + nsets + 1;
+}
+
+size_t dito_carve(DitoWs* ws, void* base, size_t cap, uint32_t nsets, uint64_t total, uint32_t tile) {
+    Carver c{(char*)base, 0, cap};
+    const size_t T = dito_max_tiles(nsets, total, tile);
+    ws->offsets_single = c.take<uint64_t>(2);
+    ws->first_tile = c.take<uint32_t>(nsets + 1);
+    ws->tile_set = c.take<uint32_t>(T);
+    ws->part1_v = c.take<float>(T * 14);
+    ws->part1_i = c.take<uint32_t>(T * 14);
+    ws->part2 = c.take<float>(T * 6);
+    ws->state = c.take<SetState>(nsets);
+    ws->trace_tmp = c.take<obvh_dito_trace>(nsets);
+    return c.used;
+}
+
+uint32_t dito_auto_tile(uint32_t nsets, uint64_t total) {
+    // about 4096 tiles for the whole job, between 512 and 65536 points each
+    (void)nsets;
+    uint64_t want = total / 4096;
+    uint64_t t = (want + kChunk - 1) / kChunk * kChunk;
+    if (t < (uint64_t)kChunk) t = kChunk;
+    if (t > 65536) t = 65536;
+    return (uint32_t)t;
+}
+
+template <class P>
+static void launch_passes(P p, const uint64_t* offs, uint32_t nsets, uint32_t tile, uint32_t workers, DitoWs ws,
+                          kr_obb3* out, obvh_dito_trace* tr, hipStream_t stream) {
+    const dim3 set_grid((nsets + kWaves - 1) / kWaves);
+    hipLaunchKernelGGL(dito_pass1<P>, dim3(workers), dim3(kBlock), 0, stream, p, offs, nsets, tile, ws);
+    hipLaunchKernelGGL(dito_cand<P>, set_grid, dim3(kBlock), 0, stream, p, offs, nsets, ws, tr);
+    hipLaunchKernelGGL(dito_pass2<P>, dim3(workers), dim3(kBlock), 0, stream, p, offs, nsets, tile, ws);
+    hipLaunchKernelGGL(dito_final, set_grid, dim3(kBlock), 0, stream, offs, nsets, ws, out, tr);
+}
+
+kr_error dito_launch(const float* x, const float* y, const float* z, const float* xyz,
+                     const uint64_t* offsets, uint64_t single_count, uint32_t nsets, uint64_t total,
+                     kr_obb3* out, obvh_dito_trace* trace, uint32_t tile, uint32_t workers,
+                     void* workspace, size_t ws_bytes, hipStream_t stream) {
+    if (nsets == 0) return nullptr;
+    if (tile == 0) tile = dito_auto_tile(nsets, total);
+    if (tile % kChunk != 0) return "obvh_dito: tile_points must be a multiple of 512";
+    DitoWs ws;
+    const size_t need = dito_carve(&ws, workspace, ws_bytes, nsets, total, tile);
+    if (workspace == nullptr || ws_bytes < need) return "obvh_dito: workspace too small";
+    const uint64_t* offs = offsets ? offsets : ws.offsets_single;
+    obvh_dito_trace* tr = trace ? trace : ws.trace_tmp;
+    const size_t T = dito_max_tiles(nsets, total, tile);
+    if (workers == 0) workers = (uint32_t)(T < 4096 ? T : 4096);
+    if (workers > T) workers = (uint32_t)T;
+    hipLaunchKernelGGL(dito_setup, dim3(1), dim3(1024), 0, stream, offsets, single_count, nsets, tile, ws, out, tr);
+    uint32_t tb = (uint32_t)((T + 255) / 256);
+    if (tb > 4096) tb = 4096;
+    hipLaunchKernelGGL(dito_tile_table, dim3(tb), dim3(256), 0, stream, nsets, ws);
+    if (xyz) launch_passes(PtsAoS{xyz}, offs, nsets, tile, workers, ws, out, tr, stream);
+    else launch_passes(PtsSoA{x, y, z}, offs, nsets, tile, workers, ws, out, tr, stream);
+    OBVH_HIP_TRY(hipGetLastError());
+    return nullptr;
+}
+
+}  // namespace obvh

@@ -1,0 +1,30 @@
+// transform.hpp -- internal interface of the transform kernels (transform.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "obvh/obvh.h"
+
+namespace obvh {
+
+// Candidate axes of an internal node (48 B).
+struct Cand {
+    float b[9];
+    int32_t degen;  // 1: the node's box is its AABB
+    int32_t pad[2];
+};
+
+struct TreeWs {
+    uint32_t* visit;   // internal_count (+1) arrival counters
+    uint32_t* error;   // 1: parent index out of range / too deep, 2: parent/child mismatch,
+                       // 4: face id out of range, 8: a node did not complete
+    float4* kdop;      // 7 per node: {min, min vertex id, max, max vertex id}
+    Cand* cand;        // per internal node
+    int32_t* range;    // 6 per internal node: order-preserving keys, min b0..b2, max b0..b2
+};
+
+size_t tree_carve(TreeWs* ws, void* base, size_t cap, uint32_t N);
+kr_error tree_launch(const kr_bvh_node* nodes, const uint32_t* parents, const kr_vec3* V,
+                     const kr_uvec4* F, kr_obb3* obbs, kr_mat4* mats, uint32_t* ids, uint32_t N,
+                     void* workspace, size_t ws_bytes, hipStream_t stream);
+
+}  // namespace obvh

@@ -1,0 +1,173 @@
+"""Python entry points over the HIP library.
+
+Mirrors the reference operator interface for the OBB path:
+
+* ``points_obb(points)``            kr_cuda_points_obb (dito.cuh:12-13): host
+                                    AoS points in, one OBB (15 floats) out.
+* ``bvh_obb_tree(...)``             kr_cuda_bvh_obb_tree (transforms.cuh:52-65):
+                                    device tree in, obbs / transforms out.
+* ``dito(x, y, z)``, ``dito_batch`` device-resident, stream-ordered DiTO.
+* ``lbvh_build(V, F)``              device LBVH (lbvh.cu:1547-1692).
+* ``obb_tree(...)``                 stream-ordered transform with workspace.
+
+Device memory and streams come from torch (plumbing only); every compute step
+runs in libobvh.so.  Errors raise ObvhError; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import ObvhError, check, lib
+from .layouts import NODE_DTYPE, TRACE_DTYPE
+
+
+def _torch():
+    import torch
+
+    if not torch.cuda.is_available():
+        raise ObvhError("no GPU visible: the OBB engine runs on the HIP device only")
+    return torch
+
+
+def _ptr(t) -> C.c_void_p:
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _stream(stream=None) -> C.c_void_p:
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def _workspace(nbytes: int):
+    torch = _torch()
+    return torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device="cuda")
+
+
+# ---------------------------------------------------------------- DiTO --
+
+def points_obb(points, return_time: bool = False):
+    """kr_cuda_points_obb on a host (n, 3) float32 array -> (15,) float32."""
+    p = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+    t = C.c_float(0.0)
+    o = lib().kr_cuda_points_obb(p.ctypes.data_as(C.c_void_p), p.shape[0], C.byref(t))
+    out = np.array([o.mid.x, o.mid.y, o.mid.z, o.v0.x, o.v0.y, o.v0.z, o.v1.x, o.v1.y, o.v1.z,
+                    o.v2.x, o.v2.y, o.v2.z, o.ext.x, o.ext.y, o.ext.z], dtype=np.float32)
+    if np.isnan(out).all() and p.shape[0] > 0:
+        err = lib().obvh_last_error()
+        raise ObvhError(err.decode() if err else "kr_cuda_points_obb failed")
+    return (out, t.value) if return_time else out
+
+
+def dito(x, y, z, trace: bool = False, stream=None, workspace=None):
+    """DiTO-14 (+0.001 extent floor) of one SoA set of device float32 tensors."""
+    torch = _torch()
+    n = int(x.numel())
+    if not (y.numel() == n and z.numel() == n):
+        raise ValueError("x, y, z must have the same length")
+    for a in (x, y, z):
+        if a.dtype != torch.float32 or not a.is_cuda or not a.is_contiguous():
+            raise ValueError("points must be contiguous float32 CUDA tensors")
+    out = torch.empty(15, dtype=torch.float32, device="cuda")
+    tr = torch.zeros(TRACE_DTYPE.itemsize, dtype=torch.uint8, device="cuda") if trace else None
+    need = lib().obvh_dito_workspace_size(n)
+    ws = workspace if workspace is not None else _workspace(need)
+    check(lib().obvh_dito_soa(_ptr(x), _ptr(y), _ptr(z), n, _ptr(out), _ptr(tr), _ptr(ws),
+                              ws.numel(), _stream(stream)))
+    if trace:
+        return out, tr
+    return out
+
+
+def dito_batch(x, y, z, offsets, tile_points: int = 0, workers: int = 0, trace: bool = False,
+               stream=None, workspace=None):
+    """Batched DiTO.  offsets: int64 CUDA tensor with nsets + 1 entries."""
+    torch = _torch()
+    nsets = int(offsets.numel()) - 1
+    total = int(x.numel())
+    if offsets.dtype != torch.int64 or not offsets.is_cuda:
+        raise ValueError("offsets must be an int64 CUDA tensor")
+    out = torch.empty((max(nsets, 0), 15), dtype=torch.float32, device="cuda")
+    tr = torch.zeros(max(nsets, 1) * TRACE_DTYPE.itemsize, dtype=torch.uint8, device="cuda") if trace else None
+    need = lib().obvh_dito_batch_workspace_size(max(nsets, 1), total, tile_points)
+    ws = workspace if workspace is not None else _workspace(need)
+    if ws.numel() < need:
+        raise ValueError(f"workspace needs {need} bytes")
+    check(lib().obvh_dito_batch(_ptr(x), _ptr(y), _ptr(z), _ptr(offsets), nsets, total, _ptr(out), _ptr(tr),
+                                tile_points, workers, _ptr(ws), ws.numel(), _stream(stream)))
+    if trace:
+        return out, tr
+    return out
+
+
+def trace_to_numpy(tr) -> np.ndarray:
+    return tr.cpu().numpy().view(TRACE_DTYPE)
+
+
+# ---------------------------------------------------------------- LBVH --
+
+def lbvh_build(V, F, stream=None):
+    """Device LBVH over (V,3) float32 vertices and (F,4) int32/uint32 faces.
+
+    Returns dict(nodes=uint8 tensor (2F-1)*36 B, parents, counts (int32
+    tensors, values are u32), costs (float32), mortons (int32, sorted)).
+    """
+    torch = _torch()
+    nf = int(F.shape[0])
+    total = 2 * nf - 1
+    nodes = torch.empty(total * NODE_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+    parents = torch.empty(total, dtype=torch.int32, device="cuda")
+    counts = torch.empty(total, dtype=torch.int32, device="cuda")
+    costs = torch.empty(total, dtype=torch.float32, device="cuda")
+    mortons = torch.empty(nf, dtype=torch.int32, device="cuda")
+    ws = _workspace(lib().obvh_lbvh_workspace_size(nf))
+    check(lib().obvh_lbvh_build(_ptr(V), _ptr(F), nf, _ptr(nodes), _ptr(parents), _ptr(counts), _ptr(costs),
+                                _ptr(mortons), _ptr(ws), ws.numel(), _stream(stream)))
+    return {"nodes": nodes, "parents": parents, "counts": counts, "costs": costs, "mortons": mortons}
+
+
+def nodes_to_numpy(nodes) -> np.ndarray:
+    return nodes.cpu().numpy().view(NODE_DTYPE)
+
+
+# ----------------------------------------------------------- transform --
+
+def obb_tree(nodes, parents, V, F, with_ids: bool = False, stream=None, workspace=None, out=None):
+    """Stream-ordered transform.  Returns (obbs (2N-1,15), mats (2N-1,16)[, ids (2N-1,14)])."""
+    torch = _torch()
+    total = int(parents.numel())
+    N = (total + 1) // 2
+    if out is None:
+        obbs = torch.empty((total, 15), dtype=torch.float32, device="cuda")
+        mats = torch.empty((total, 16), dtype=torch.float32, device="cuda")
+    else:
+        obbs, mats = out
+    ids = torch.empty((total, 14), dtype=torch.int32, device="cuda") if with_ids else None
+    need = lib().obvh_obb_tree_workspace_size(N, N - 1)
+    ws = workspace if workspace is not None else _workspace(need)
+    if ws.numel() < need:
+        raise ValueError(f"workspace needs {need} bytes")
+    check(lib().obvh_obb_tree_async(_ptr(nodes), _ptr(parents), _ptr(V), _ptr(F), _ptr(obbs), _ptr(mats),
+                                    _ptr(ids), N, N - 1, _ptr(ws), ws.numel(), _stream(stream)))
+    return (obbs, mats, ids) if with_ids else (obbs, mats)
+
+
+def bvh_obb_tree(nodes, parents, V, F, obbs, mats, primitive_count: int, internal_count: int,
+                 primitive_counts=None, costs=None, primitives=None, ci: float = 1.2, ct: float = 1.0,
+                 obb_cost: float = 1.0, primitive_threshold: int = 0xFFFFFFFF) -> None:
+    """kr_cuda_bvh_obb_tree (legacy, synchronous; all arguments device tensors)."""
+    err = lib().kr_cuda_bvh_obb_tree(_ptr(nodes), _ptr(parents), _ptr(primitive_counts), _ptr(costs), _ptr(V),
+                                     _ptr(F), _ptr(primitives), _ptr(obbs), _ptr(mats), ci, ct, obb_cost,
+                                     primitive_count, internal_count, primitive_threshold)
+    check(err)
+
+
+def validate_tree(nodes: np.ndarray, parents: np.ndarray) -> None:
+    """Host-side structural check of a tree (raises ObvhError)."""
+    nodes = np.ascontiguousarray(nodes, dtype=NODE_DTYPE)
+    parents = np.ascontiguousarray(parents).view(np.uint32)
+    N = (nodes.shape[0] + 1) // 2
+    check(lib().obvh_validate_tree_host(nodes.ctypes.data_as(C.c_void_p), parents.ctypes.data_as(C.c_void_p),
+                                        N, N - 1))

@@ -1,0 +1,155 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, __graft_entry__.smoke() and
+the cpu_baseline leg of bench.py, as the checker -- never by the product
+path (obvh_amd), which must fail loudly when the HIP library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from obvh_amd.layouts import NODE_DTYPE, TRACE_DTYPE
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (gcc, no FMA contraction)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE, "-B" if force else "all"], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        vp = C.c_void_p
+        L.oracle_dito14.argtypes = [vp, C.c_int, vp, vp]
+        L.oracle_points_obb.argtypes = [vp, C.c_int, vp, vp]
+        L.oracle_points_obb_batch.argtypes = [vp, vp, C.c_int, vp, vp]
+        L.oracle_obb_tree.argtypes = [vp, vp, vp, vp, C.c_uint32, vp, vp, vp, vp]
+        L.oracle_obb_tree.restype = C.c_int
+        L.oracle_lbvh_build.argtypes = [vp, vp, C.c_uint32, vp, vp, vp, vp, vp]
+        L.oracle_lbvh_build.restype = C.c_int
+        L.oracle_cpu_obb_tree.argtypes = [vp, vp, vp, vp, C.c_uint32, vp, vp, C.c_uint32, C.c_uint32]
+        L.oracle_cpu_obb_tree.restype = C.c_int64
+        L.oracle_minverse4.argtypes = [vp, vp]
+        L.oracle_mmul4.argtypes = [vp, vp, vp]
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def dito14(points, trace: bool = False):
+    """DiTO_14<float> (no extent floor).  Returns obb (15,) [, trace]."""
+    p = _f32(points).reshape(-1, 3)
+    out = np.zeros(15, dtype=np.float32)
+    tr = np.zeros(1, dtype=TRACE_DTYPE)
+    lib().oracle_dito14(_p(p), int(p.shape[0]), _p(out), _p(tr))
+    return (out, tr[0]) if trace else out
+
+
+def points_obb(points, trace: bool = False):
+    """kr_points_obb: DiTO_14 + 0.001 extent floor.  Returns obb (15,) [, trace]."""
+    p = _f32(points).reshape(-1, 3)
+    out = np.zeros(15, dtype=np.float32)
+    tr = np.zeros(1, dtype=TRACE_DTYPE)
+    lib().oracle_points_obb(_p(p), int(p.shape[0]), _p(out), _p(tr))
+    return (out, tr[0]) if trace else out
+
+
+def points_obb_batch(points, offsets):
+    p = _f32(points).reshape(-1, 3)
+    off = np.ascontiguousarray(offsets, dtype=np.uint64)
+    n = off.size - 1
+    out = np.zeros((n, 15), dtype=np.float32)
+    tr = np.zeros(n, dtype=TRACE_DTYPE)
+    lib().oracle_points_obb_batch(_p(p), _p(off), n, _p(out), _p(tr))
+    return out, tr
+
+
+def lbvh_build(vertices, faces):
+    """Returns (nodes, parents, counts, costs, mortons_sorted)."""
+    V = _f32(vertices).reshape(-1, 3)
+    F = np.ascontiguousarray(faces, dtype=np.uint32).reshape(-1, 4)
+    n = F.shape[0]
+    nodes = np.zeros(2 * n - 1, dtype=NODE_DTYPE)
+    parents = np.zeros(2 * n - 1, dtype=np.uint32)
+    counts = np.zeros(2 * n - 1, dtype=np.uint32)
+    costs = np.zeros(2 * n - 1, dtype=np.float32)
+    mortons = np.zeros(n, dtype=np.uint32)
+    rc = lib().oracle_lbvh_build(_p(V), _p(F), n, _p(nodes), _p(parents), _p(counts), _p(costs), _p(mortons))
+    if rc != 0:
+        raise RuntimeError("oracle_lbvh_build failed")
+    return nodes, parents, counts, costs, mortons
+
+
+def obb_tree(nodes, parents, vertices, faces, with_ids: bool = False, with_ranges: bool = False):
+    """The transform method.  Returns obbs (2N-1, 15), mats (2N-1, 16) [, ids (2N-1, 14)] [, ranges]."""
+    V = _f32(vertices).reshape(-1, 3)
+    F = np.ascontiguousarray(faces, dtype=np.uint32).reshape(-1, 4)
+    nodes = np.ascontiguousarray(nodes, dtype=NODE_DTYPE)
+    parents = np.ascontiguousarray(parents, dtype=np.uint32)
+    total = nodes.shape[0]
+    N = (total + 1) // 2
+    obbs = np.zeros((total, 15), dtype=np.float32)
+    mats = np.zeros((total, 16), dtype=np.float32)
+    ids = np.zeros((total, 14), dtype=np.uint32) if with_ids else None
+    rr = np.zeros((max(N - 1, 1), 6), dtype=np.float32) if with_ranges else None
+    rc = lib().oracle_obb_tree(
+        _p(nodes), _p(parents), _p(V), _p(F), N, _p(obbs), _p(mats),
+        _p(ids) if ids is not None else None, _p(rr) if rr is not None else None)
+    if rc != 0:
+        raise ValueError("oracle_obb_tree: malformed tree")
+    res = [obbs, mats]
+    if with_ids:
+        res.append(ids)
+    if with_ranges:
+        res.append(rr)
+    return tuple(res)
+
+
+def cpu_obb_tree(nodes, parents, vertices, faces, leaf_begin: int = 0, leaf_end: int | None = None):
+    """Reference CPU transform (full DiTO per node).  Returns (obbs, mats, nodes_out, fitted)."""
+    V = _f32(vertices).reshape(-1, 3)
+    F = np.ascontiguousarray(faces, dtype=np.uint32).reshape(-1, 4)
+    nodes = np.array(nodes, dtype=NODE_DTYPE, copy=True)
+    parents = np.ascontiguousarray(parents, dtype=np.uint32)
+    total = nodes.shape[0]
+    N = (total + 1) // 2
+    obbs = np.zeros((total, 15), dtype=np.float32)
+    mats = np.zeros((total, 16), dtype=np.float32)
+    le = N if leaf_end is None else leaf_end
+    fitted = lib().oracle_cpu_obb_tree(_p(nodes), _p(parents), _p(V), _p(F), N, _p(obbs), _p(mats), leaf_begin, le)
+    return obbs, mats, nodes, int(fitted)
+
+
+def minverse4(m):
+    m = _f32(m).reshape(16)
+    out = np.zeros(16, dtype=np.float32)
+    lib().oracle_minverse4(_p(m), _p(out))
+    return out
+
+
+def mmul4(a, b):
+    a = _f32(a).reshape(16)
+    b = _f32(b).reshape(16)
+    out = np.zeros(16, dtype=np.float32)
+    lib().oracle_mmul4(_p(a), _p(b), _p(out))
+    return out

@@ -1,0 +1,91 @@
+/*
+ * oracle.h -- CPU restatement of the reference's OBB algorithms.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library; it is the checker,
+ * never the product path.
+ *
+ * Build: gcc -O2 -ffp-contract=off (no FMA contraction, x86-64 SSE math,
+ * correctly rounded / and sqrt), see oracle/Makefile.
+ *
+ * Contents and the reference code each part restates (paths relative to
+ * the reference repository root):
+ *   dito.c           DiTO_14<float> + kr_points_obb
+ *                    src/common/algorithm/dito/dito.cpp:56-538,
+ *                    src/common/geometry.cpp:232-252
+ *   transform.c      the GPU transform method K1-K4
+ *                    src/ads/common/cuda/transforms.cu:213-964
+ *   lbvh.c           LBVH build (centroid box, 30-bit Morton, stable sort,
+ *                    Karras split, bottom-up AABBs/counts/costs)
+ *                    src/ads/lbvh_gpu/lbvh.cu:252-457,1547-1692,
+ *                    src/ads/common/cuda/transforms.cu:1127-1250
+ *   cpu_transform.c  the reference CPU transform kr_bvh_obb_tree (full
+ *                    DiTO per node), src/ads/common/transforms.cpp:349-481
+ */
+#ifndef OBVH_ORACLE_H
+#define OBVH_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { float x, y, z; } ov3;
+typedef struct { ov3 mid, v0, v1, v2, ext; } oobb;
+typedef struct { float v[16]; } omat4;
+typedef struct {
+    ov3 bmin, bmax;
+    uint32_t a, b;          /* leaf: a = face id; internal: a = left, b = right */
+    uint16_t nprim;
+    uint8_t axis, pad;
+} onode;                     /* 36 B, same layout as kr_bvh_node */
+
+typedef struct {
+    uint32_t slab_min_idx[7];
+    uint32_t slab_max_idx[7];
+    int32_t pair_k, p2_sel, q0_sel, q1_sel, q0_valid, q1_valid;
+    int32_t degenerate, best, result_aabb, pad_;
+} odito_trace;               /* 96 B, same layout as obvh_dito_trace */
+
+/* DiTO_14<float> on AoS xyz (n points).  trace may be NULL. */
+void oracle_dito14(const float* xyz, int n, oobb* out, odito_trace* trace);
+/* kr_points_obb: DiTO_14 followed by ext = max(ext, 0.001). */
+void oracle_points_obb(const float* xyz, int n, oobb* out, odito_trace* trace);
+/* Many sets back to back: set s = points [off[s], off[s+1]). */
+void oracle_points_obb_batch(const float* xyz, const uint64_t* off, int nsets,
+                             oobb* out, odito_trace* traces);
+
+/* The GPU transform method.  nodes/parents: 2N-1 entries (N = nprim);
+ * outputs obbs/mats for every node.  kdop_ids (nullable): 14 ids per node
+ * [min0..min6, max0..max6].  refit_ranges (nullable): 6 floats per node
+ * (min b0..b2, max b0..b2 after refit; internal nodes only).
+ * Returns 0 on success, -1 on a malformed tree. */
+int oracle_obb_tree(const onode* nodes, const uint32_t* parents,
+                    const float* vertices, const uint32_t* faces /* uvec4 */,
+                    uint32_t nprim, oobb* obbs, omat4* mats,
+                    uint32_t* kdop_ids, float* refit_ranges);
+
+/* LBVH build.  vertices AoS xyz, faces uvec4.  Outputs 2N-1 nodes, parents,
+ * primitive_counts, costs; mortons_sorted (N entries, nullable). */
+int oracle_lbvh_build(const float* vertices, const uint32_t* faces, uint32_t nfaces,
+                      onode* nodes, uint32_t* parents, uint32_t* counts,
+                      float* costs, uint32_t* mortons_sorted);
+
+/* Reference CPU transform kr_bvh_obb_tree (DiTO of every subtree).
+ * Processes leaves [leaf_begin, leaf_end) only (the reference climbs from
+ * every leaf; a sub-range bounds the CPU time of a timed sample).
+ * Returns the number of internal nodes it fitted. */
+int64_t oracle_cpu_obb_tree(onode* nodes, const uint32_t* parents,
+                            const float* vertices, const uint32_t* faces,
+                            uint32_t nprim, oobb* obbs, omat4* mats,
+                            uint32_t leaf_begin, uint32_t leaf_end);
+
+/* 4x4 helpers exposed for tests (vecmath.h:946-1075,1214-1234). */
+void oracle_minverse4(const omat4* m, omat4* out);
+void oracle_mmul4(const omat4* l, const omat4* r, omat4* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

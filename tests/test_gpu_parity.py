@@ -1,0 +1,203 @@
+"""GPU parity: every HIP entry point against the CPU oracle on the same
+seeded inputs.  Integer/index results must be bit-exact; float results are
+compared bit-exactly too (== on values, so -0 == +0), which is stricter than
+the 1e-5 relative bound the path is specified to.  Full-size runs are
+checked through size-independent properties (enclosure, determinism)."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from obvh_amd import ops, synth  # noqa: E402
+from obvh_amd.layouts import NODE_DTYPE, TRACE_DTYPE  # noqa: E402
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+
+    if not t.cuda.is_available():
+        pytest.skip("no GPU")
+    return t
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def to_dev(torch, a, dtype=None):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+
+
+def soa(torch, p):
+    p = np.ascontiguousarray(p, np.float32)
+    return [torch.from_numpy(np.ascontiguousarray(p[:, k])).cuda() for k in range(3)]
+
+
+TRACE_INT_FIELDS = ["slab_min_idx", "slab_max_idx", "pair_k", "p2_sel", "q0_sel", "q1_sel", "q0_valid",
+                    "q1_valid", "degenerate", "best", "result_aabb"]
+
+
+def assert_trace_equal(got, exp):
+    for f in TRACE_INT_FIELDS:
+        assert np.array_equal(got[f], exp[f]), f
+
+
+DITO_CASES = {
+    "gaussian131072": lambda: synth.gaussian_points(131072, seed=1),
+    "gaussian1000": lambda: synth.gaussian_points(1000, seed=2),
+    "n1": lambda: np.array([[1.0, 2.0, 3.0]], np.float32),
+    "n3": lambda: synth.gaussian_points(3, seed=3),
+    "n14": lambda: synth.gaussian_points(14, seed=4),
+    "n15": lambda: synth.gaussian_points(15, seed=5),
+    "dups": lambda: (np.random.default_rng(6).integers(-8, 8, (5000, 3)) * 0.25).astype(np.float32),
+    "line": lambda: (np.linspace(-1, 1, 777)[:, None] * np.array([0.3, -0.5, 0.8])).astype(np.float32),
+    "point": lambda: np.tile(np.array([[0.5, 0.5, 0.5]], np.float32), (600, 1)),
+    "ragged": lambda: synth.gaussian_points(512 * 7 + 13, seed=7),
+}
+
+
+@pytest.mark.parametrize("case", sorted(DITO_CASES))
+def test_dito_soa_matches_oracle(torch, oracle_lib, case):
+    say(f"[dito_soa] case={case}")
+    p = DITO_CASES[case]()
+    exp, etr = oracle_lib.points_obb(p, trace=True)
+    x, y, z = soa(torch, p)
+    out, tr = ops.dito(x, y, z, trace=True)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    assert np.array_equal(got, exp), (got, exp)
+    assert_trace_equal(ops.trace_to_numpy(tr)[0], etr)
+
+
+@pytest.mark.parametrize("case", ["gaussian131072", "n3", "dups", "point"])
+def test_legacy_points_obb_matches_oracle(torch, oracle_lib, case):
+    say(f"[kr_cuda_points_obb] case={case}")
+    p = DITO_CASES[case]()
+    got, ms = ops.points_obb(p, return_time=True)
+    assert np.array_equal(got, oracle_lib.points_obb(p))
+    assert ms >= 0.0
+
+
+@pytest.mark.parametrize("tile", [0, 512, 4096])
+def test_dito_batch_matches_oracle(torch, oracle_lib, tile):
+    say(f"[dito_batch] tile={tile}")
+    sizes = np.concatenate([[0, 1, 2, 14, 15, 64, 513], synth.batch_sizes(120, lo=64, hi=1 << 15, seed=9)])
+    pts, off = synth.batch_sets(sizes, seed=10)
+    exp, etr = oracle_lib.points_obb_batch(pts, off)
+    x, y, z = soa(torch, pts)
+    offs = torch.from_numpy(off.astype(np.int64)).cuda()
+    out, tr = ops.dito_batch(x, y, z, offs, tile_points=tile, trace=True)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    gtr = ops.trace_to_numpy(tr)[: len(sizes)]
+    for s in range(len(sizes)):
+        assert np.array_equal(got[s], exp[s]), (s, sizes[s])
+        assert_trace_equal(gtr[s], etr[s])
+
+
+def build_gpu_tree(torch, V, F):
+    Vd = torch.from_numpy(V).cuda()
+    Fd = torch.from_numpy(F.view(np.int32)).cuda()
+    t = ops.lbvh_build(Vd, Fd)
+    return Vd, Fd, t
+
+
+MESHES = {
+    "ico16": lambda: synth.icosphere(16, seed=1),
+    "ico_n256": lambda: synth.icosphere(8, seed=2),  # 1280 faces = 5 * 256
+    "hair200": lambda: synth.hairball(tubes=200, seed=3),
+    "single": lambda: (np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32),
+                       np.array([[0, 1, 2, 0]], np.uint32)),
+    "two": lambda: (np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], np.float32),
+                    np.array([[0, 1, 2, 0], [0, 1, 3, 0]], np.uint32)),
+}
+
+
+@pytest.mark.parametrize("mesh", sorted(MESHES))
+def test_lbvh_matches_oracle(torch, oracle_lib, mesh):
+    say(f"[lbvh] mesh={mesh}")
+    V, F = MESHES[mesh]()
+    Vd, Fd, t = build_gpu_tree(torch, V, F)
+    torch.cuda.synchronize()
+    en, ep, ec, eco, em = oracle_lib.lbvh_build(V, F)
+    gn = ops.nodes_to_numpy(t["nodes"])
+    assert np.array_equal(t["mortons"].cpu().numpy().view(np.uint32), em)
+    assert np.array_equal(t["parents"].cpu().numpy().view(np.uint32), ep)
+    assert np.array_equal(t["counts"].cpu().numpy().view(np.uint32), ec)
+    assert np.array_equal(t["costs"].cpu().numpy(), eco)
+    for f in ("bmin", "bmax", "a", "nprim", "axis"):
+        assert np.array_equal(gn[f], en[f]), f
+    I = len(F) - 1
+    assert np.array_equal(gn["b"][:I], en["b"][:I])
+
+
+@pytest.mark.parametrize("mesh", sorted(MESHES))
+def test_obb_tree_matches_oracle(torch, oracle_lib, mesh):
+    say(f"[obb_tree] mesh={mesh}")
+    V, F = MESHES[mesh]()
+    Vd, Fd, t = build_gpu_tree(torch, V, F)
+    obbs, mats, ids = ops.obb_tree(t["nodes"], t["parents"], Vd, Fd, with_ids=True)
+    torch.cuda.synchronize()
+    nodes = ops.nodes_to_numpy(t["nodes"])
+    parents = t["parents"].cpu().numpy().view(np.uint32)
+    eo, em, ei = oracle_lib.obb_tree(nodes, parents, V, F, with_ids=True)
+    assert np.array_equal(ids.cpu().numpy().view(np.uint32), ei)
+    go, gm = obbs.cpu().numpy(), mats.cpu().numpy()
+    bad = np.where(~np.all(go == eo, axis=1))[0]
+    assert bad.size == 0, (bad[:10], go[bad[:3]], eo[bad[:3]])
+    assert np.array_equal(gm, em)
+
+
+def test_legacy_obb_tree_equals_async(torch, oracle_lib):
+    say("[kr_cuda_bvh_obb_tree] ico16")
+    V, F = synth.icosphere(16, seed=4)
+    Vd, Fd, t = build_gpu_tree(torch, V, F)
+    total = 2 * len(F) - 1
+    o1 = torch.zeros((total, 15), dtype=torch.float32, device="cuda")
+    m1 = torch.zeros((total, 16), dtype=torch.float32, device="cuda")
+    ops.bvh_obb_tree(t["nodes"], t["parents"], Vd, Fd, o1, m1, len(F), len(F) - 1,
+                     primitive_counts=t["counts"], costs=t["costs"])
+    o2, m2 = ops.obb_tree(t["nodes"], t["parents"], Vd, Fd)
+    torch.cuda.synchronize()
+    assert torch.equal(o1, o2) and torch.equal(m1, m2)
+
+
+def test_legacy_obb_tree_rejects_malformed_parents(torch):
+    say("[kr_cuda_bvh_obb_tree] malformed parents")
+    V, F = synth.icosphere(4, seed=5)
+    Vd, Fd, t = build_gpu_tree(torch, V, F)
+    parents = t["parents"].clone()
+    parents[7] = 3  # disagrees with the children of node 3 (bounds stay valid)
+    total = 2 * len(F) - 1
+    o = torch.zeros((total, 15), dtype=torch.float32, device="cuda")
+    m = torch.zeros((total, 16), dtype=torch.float32, device="cuda")
+    from obvh_amd._lib import ObvhError
+
+    with pytest.raises(ObvhError):
+        ops.bvh_obb_tree(t["nodes"], parents, Vd, Fd, o, m, len(F), len(F) - 1)
+
+
+def test_full_size_transform_properties(torch, oracle_lib):
+    # 1 003 520-triangle icosphere (f = 224): determinism across runs and
+    # parity with the oracle on the whole tree (the oracle takes seconds).
+    say("[obb_tree] ico224 full size")
+    V, F = synth.icosphere(224, seed=2)
+    Vd, Fd, t = build_gpu_tree(torch, V, F)
+    o1, m1 = ops.obb_tree(t["nodes"], t["parents"], Vd, Fd)
+    o2, m2 = ops.obb_tree(t["nodes"], t["parents"], Vd, Fd)
+    torch.cuda.synchronize()
+    assert torch.equal(o1, o2) and torch.equal(m1, m2)
+    nodes = ops.nodes_to_numpy(t["nodes"])
+    parents = t["parents"].cpu().numpy().view(np.uint32)
+    en, ep, *_ = oracle_lib.lbvh_build(V, F)
+    assert np.array_equal(parents, ep)
+    eo, em = oracle_lib.obb_tree(nodes, parents, V, F)
+    go = o1.cpu().numpy()
+    mism = int((~np.all(go == eo, axis=1)).sum())
+    assert mism == 0, mism
+    assert np.array_equal(m1.cpu().numpy(), em)

@@ -1,0 +1,284 @@
+"""Pins of the transform and LBVH oracles (oracle/transform.c, oracle/lbvh.c)
+by brute force over subtrees, an independent radix-tree definition, closed
+forms and the unit-cube contract of the output matrices."""
+import numpy as np
+import pytest
+
+from obvh_amd import synth
+
+INV = 0xFFFFFFFF
+
+
+def small_mesh(kind: str):
+    if kind == "ico":
+        return synth.icosphere(6, seed=1)
+    if kind == "hair":
+        return synth.hairball(tubes=40, seed=2)
+    raise ValueError(kind)
+
+
+def subtree_leaves(nodes, v, I):
+    """Leaves of v in left-to-right DFS order."""
+    out, stack = [], [v]
+    while stack:
+        u = stack.pop()
+        if u >= I:
+            out.append(u)
+        else:
+            stack.append(int(nodes[u]["b"]))
+            stack.append(int(nodes[u]["a"]))
+    return out
+
+
+def slab7(P):
+    x, y, z = P[:, 0], P[:, 1], P[:, 2]
+    return np.stack([x, y, z, (x + y) + z, (x + y) - z, (x - y) + z, (x - y) - z], axis=1)
+
+
+@pytest.fixture(scope="module", params=["ico", "hair"])
+def tree(request, oracle_lib):
+    V, F = small_mesh(request.param)
+    nodes, parents, counts, costs, mortons = oracle_lib.lbvh_build(V, F)
+    obbs, mats, ids, ranges = oracle_lib.obb_tree(nodes, parents, V, F, with_ids=True, with_ranges=True)
+    return dict(V=V, F=F, nodes=nodes, parents=parents, counts=counts, costs=costs, mortons=mortons,
+                obbs=obbs, mats=mats, ids=ids, ranges=ranges)
+
+
+# ---------------------------------------------------------------- LBVH --
+
+def radix_tree_reference(keys):
+    """Radix tree over the extended keys (code, index), built top-down by the
+    definition: a node over [a, b] splits where the highest bit of
+    key[a] ^ key[b] changes.  Returns {(a, b): (left_range, right_range)}."""
+    n = len(keys)
+    ext = [(int(k) << 32) | i for i, k in enumerate(keys)]
+    out = {}
+    stack = [(0, n - 1)]
+    while stack:
+        a, b = stack.pop()
+        if a == b:
+            continue
+        hb = (ext[a] ^ ext[b]).bit_length() - 1
+        split = a
+        for i in range(a, b):
+            if ((ext[i] ^ ext[a]) >> hb) == 0:
+                split = i
+        out[(a, b)] = ((a, split), (split + 1, b))
+        stack.append((a, split))
+        stack.append((split + 1, b))
+    return out
+
+
+def morton_naive(c):
+    q = [int(min(max(float(v) * 1024.0, 0.0), 1023.0)) for v in c]
+    code = 0
+    for bit in range(10):
+        for axis in range(3):
+            code |= ((q[axis] >> bit) & 1) << (3 * bit + (2 - axis))
+    return code
+
+
+def test_lbvh_leaves_are_stably_sorted(tree):
+    m = tree["mortons"]
+    assert np.all(np.diff(m.astype(np.int64)) >= 0)
+    I = len(m) - 1
+    prims = tree["nodes"]["a"][I:]
+    # stable: equal codes keep face order
+    for j in range(1, len(m)):
+        if m[j] == m[j - 1]:
+            assert prims[j] > prims[j - 1]
+    assert np.array_equal(np.sort(prims), np.arange(len(m)))
+
+
+def test_lbvh_morton_codes_naive_interleave(tree):
+    # Bit interleave by a naive loop over bits (independent of the magic
+    # masks), codes of the offset triangle-box centres in the centroid box
+    # (which also covers vertices[0] when the face count is not a multiple of 256).
+    V, F = tree["V"], tree["F"]
+    tri = V[F[:, :3]]
+    bmin, bmax = tri.min(1), tri.max(1)
+    cen = (bmin + bmax) * np.float32(0.5)
+    cmin, cmax = cen.min(0), cen.max(0)
+    if len(F) % 256:
+        cmin, cmax = np.minimum(cmin, V[0]), np.maximum(cmax, V[0])
+    c = ((bmin - cmin) + (bmax - cmin)) * np.float32(0.5)
+    ext = cmax - cmin
+    c = np.where(ext != 0, c / np.where(ext != 0, ext, 1), 0).astype(np.float32)
+    codes = np.array([morton_naive(ci) for ci in c], dtype=np.uint32)
+    I = len(F) - 1
+    prims = tree["nodes"]["a"][I:]
+    assert np.array_equal(codes[prims], tree["mortons"])
+
+
+def test_lbvh_topology_matches_radix_tree_definition(tree):
+    nodes, parents, m = tree["nodes"], tree["parents"], tree["mortons"]
+    n = len(m)
+    I = n - 1
+    ref = radix_tree_reference(m)
+
+    def rng(v):
+        if v >= I:
+            return (v - I, v - I)
+        ls = subtree_leaves(nodes, v, I)
+        return (ls[0] - I, ls[-1] - I)
+
+    assert parents[0] == INV
+    for v in range(I):
+        a, b = rng(v)
+        l, r = int(nodes[v]["a"]), int(nodes[v]["b"])
+        assert (rng(l), rng(r)) == ref[(a, b)]
+        assert parents[l] == v and parents[r] == v
+
+
+def test_lbvh_bounds_counts_costs(tree):
+    nodes, counts, costs = tree["nodes"], tree["counts"], tree["costs"]
+    V, F = tree["V"], tree["F"]
+    I = len(F) - 1
+
+    def area(mn, mx):
+        d = (mx - mn).astype(np.float32)
+        return np.float32(2) * (d[0] * d[1] + d[0] * d[2] + d[1] * d[2])
+
+    for v in range(2 * len(F) - 1):
+        ls = subtree_leaves(nodes, v, I)
+        pts = V[F[nodes["a"][ls]][:, :3].ravel()]
+        assert np.array_equal(nodes[v]["bmin"], pts.min(0))
+        assert np.array_equal(nodes[v]["bmax"], pts.max(0))
+        assert counts[v] == len(ls)
+        if v >= I:
+            assert costs[v] == np.float32(1.0) * area(nodes[v]["bmin"], nodes[v]["bmax"])
+        else:
+            l, r = nodes[v]["a"], nodes[v]["b"]
+            expect = np.float32(1.2) * area(nodes[v]["bmin"], nodes[v]["bmax"]) + costs[r] + costs[l]
+            assert costs[v] == expect
+
+
+# ------------------------------------------------------------ transform --
+
+def test_kdop_ids_are_first_extremes_in_dfs_order(tree):
+    # Brute force: over the subtree's vertices in DFS leaf order and face
+    # vertex order, the first vertex attaining each slab min / max.
+    nodes, V, F, ids = tree["nodes"], tree["V"], tree["F"], tree["ids"]
+    I = len(F) - 1
+    for v in range(2 * len(F) - 1):
+        ls = subtree_leaves(nodes, v, I)
+        vid = F[nodes["a"][ls]][:, :3].ravel()
+        pr = slab7(V[vid])
+        assert np.array_equal(ids[v, :7], vid[pr.argmin(0)])
+        assert np.array_equal(ids[v, 7:], vid[pr.argmax(0)])
+
+
+def test_refit_ranges_are_exact_subtree_ranges(tree):
+    nodes, V, F, obbs, ranges = tree["nodes"], tree["V"], tree["F"], tree["obbs"], tree["ranges"]
+    I = len(F) - 1
+    for v in range(I):
+        ax = obbs[v, 3:12].reshape(3, 3)
+        if np.array_equal(ax, np.eye(3, dtype=np.float32)):
+            continue  # the AABB branch
+        ls = subtree_leaves(nodes, v, I)
+        P = V[F[nodes["a"][ls]][:, :3].ravel()]
+        # float32 dot in the reference order, exact
+        d = (P[:, None, 0] * ax[None, :, 0] + P[:, None, 1] * ax[None, :, 1]) + P[:, None, 2] * ax[None, :, 2]
+        assert np.array_equal(ranges[v, :3], d.min(0))
+        assert np.array_equal(ranges[v, 3:], d.max(0))
+
+
+def test_matrices_map_subtree_into_unit_cube(tree):
+    # Traversal contract (vecmath.h:1719-1756): M * v lies in [-0.5, 0.5]^3.
+    nodes, V, F, obbs, mats = tree["nodes"], tree["V"], tree["F"], tree["obbs"], tree["mats"]
+    I = len(F) - 1
+    for v in range(2 * len(F) - 1):
+        M = mats[v].astype(np.float64).reshape(4, 4).T  # column major
+        ls = subtree_leaves(nodes, v, I)
+        P = V[F[nodes["a"][ls]][:, :3].ravel()].astype(np.float64)
+        L = P @ M[:3, :3].T + M[:3, 3]
+        # float32 rounding of the matrix (built and inverted in float32 by the
+        # reference formulas): a few dozen ulp of the magnitudes each local
+        # coordinate is summed from
+        mag = np.abs(P) @ np.abs(M[:3, :3]).T + np.abs(M[:3, 3])
+        slack = 64 * np.finfo(np.float32).eps * mag
+        assert np.all(np.abs(L) <= 0.5 + 1e-6 + slack), v
+
+
+def test_internal_axes_orthonormal_and_leaf_frame(tree):
+    obbs, F = tree["obbs"], tree["F"]
+    I = len(F) - 1
+    ax = obbs[:I, 3:12].reshape(-1, 3, 3).astype(np.float64)
+    g = np.einsum("nij,nkj->nik", ax, ax)
+    # m = e x n is not renormalised (transforms.cu:495-497): nearly parallel
+    # base-triangle edges cost a few digits, so the bound is loose
+    assert np.allclose(g, np.eye(3)[None], atol=1e-3)
+    assert np.mean(np.abs(g - np.eye(3)[None]).max(axis=(1, 2)) < 1e-5) > 0.99
+    # leaves: v0 = unit edge, v2 = unnormalised face normal (transforms.cu:803-812)
+    V = tree["V"]
+    for i in range(0, len(F), 7):
+        f = F[tree["nodes"]["a"][I + i]]
+        e0 = V[f[1]].astype(np.float64) - V[f[0]]
+        e1 = V[f[2]].astype(np.float64) - V[f[0]]
+        o = obbs[I + i, 3:12].reshape(3, 3)
+        assert np.allclose(o[0], e0 / np.linalg.norm(e0), atol=1e-6)
+        tol = 8 * np.finfo(np.float32).eps * np.linalg.norm(e0) * np.linalg.norm(e1)
+        assert np.all(np.abs(o[2] - np.cross(e0, e1)) <= tol)
+
+
+def test_planar_mesh_gives_aabbs(oracle_lib):
+    # All vertices in z = 0: both tetrahedron apexes are invalid for every
+    # internal node (transforms.cu:646-647), so every internal box is its AABB.
+    n = 12
+    g = np.stack(np.meshgrid(np.arange(n), np.arange(n), indexing="ij"), -1).reshape(-1, 2)
+    V = np.zeros((n * n, 3), np.float32)
+    V[:, :2] = g * np.float32(0.37)
+    V[:, :2] += np.random.default_rng(1).uniform(-0.1, 0.1, size=(n * n, 2)).astype(np.float32)
+    q = (np.arange(n - 1)[:, None] * n + np.arange(n - 1)[None, :]).ravel()
+    F = np.zeros((2 * q.size, 4), np.uint32)
+    F[0::2, :3] = np.stack([q, q + n, q + 1], 1)
+    F[1::2, :3] = np.stack([q + 1, q + n, q + n + 1], 1)
+    nodes, parents, *_ = oracle_lib.lbvh_build(V, F)
+    obbs, mats = oracle_lib.obb_tree(nodes, parents, V, F)
+    I = len(F) - 1
+    assert np.array_equal(obbs[:I, 3:12], np.tile(np.eye(3, dtype=np.float32).ravel(), (I, 1)))
+    ext = obbs[:I, 12:]
+    assert np.array_equal(ext[:, 2], np.full(I, 0.001, np.float32))
+    half = (nodes["bmax"][:I] - nodes["bmin"][:I]) * np.float32(0.5)
+    assert np.array_equal(ext[:, :2], np.maximum(half[:, :2], np.float32(0.001)))
+
+
+def test_malformed_tree_is_rejected(oracle_lib, tree):
+    nodes = tree["nodes"].copy()
+    parents = tree["parents"].copy()
+    parents[5] = 7  # breaks parent/child agreement
+    with pytest.raises(ValueError):
+        oracle_lib.obb_tree(nodes, parents, tree["V"], tree["F"])
+
+
+def test_matrix_helpers_against_numpy(oracle_lib):
+    rng = np.random.default_rng(2)
+    for _ in range(50):
+        A = rng.uniform(-2, 2, (4, 4)).astype(np.float32) + 4 * np.eye(4, dtype=np.float32)
+        B = rng.uniform(-2, 2, (4, 4)).astype(np.float32)
+        colA, colB = A.T.ravel(), B.T.ravel()
+        prod = oracle_lib.mmul4(colA, colB).reshape(4, 4).T
+        assert np.allclose(prod, A.astype(np.float64) @ B, rtol=1e-5, atol=1e-5)
+        inv = oracle_lib.minverse4(colA).reshape(4, 4).T
+        assert np.allclose(inv, np.linalg.inv(A.astype(np.float64)), rtol=1e-4, atol=1e-5)
+    sing = np.zeros(16, np.float32)
+    assert np.array_equal(oracle_lib.minverse4(sing), np.eye(4, dtype=np.float32).ravel())
+
+
+def test_reference_cpu_transform_matches_dito_of_subtree(oracle_lib, tree):
+    # kr_bvh_obb_tree (transforms.cpp:371-481): every node's box is
+    # kr_points_obb of its subtree's vertices in breadth-first leaf order.
+    nodes, parents, V, F = tree["nodes"], tree["parents"], tree["V"], tree["F"]
+    obbs, mats, nodes_out, fitted = oracle_lib.cpu_obb_tree(nodes, parents, V, F)
+    I = len(F) - 1
+    assert fitted == I
+    for v in range(0, I, 5):
+        queue, order = [v], []
+        while queue:
+            u = queue.pop(0)
+            if u >= I:
+                order.append(u)
+            else:
+                queue += [int(nodes[u]["a"]), int(nodes[u]["b"])]
+        P = V[F[nodes["a"][order]][:, :3].ravel()]
+        assert np.array_equal(obbs[v], oracle_lib.points_obb(P))
