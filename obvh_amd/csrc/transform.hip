@@ -33,6 +33,7 @@
 namespace obvh {
 
 constexpr uint32_t kInvalid = 0xFFFFFFFFu;
+constexpr uint32_t kClimbChunk = 4096;  // leaves per phase-1 workgroup
 
 struct Mat4 {
     float v[16];
@@ -174,68 +175,140 @@ __device__ __forceinline__ void leaf_obb(f3 v0, f3 v1, f3 v2, kr_obb3* obb, kr_m
     store_mat(mat, M);
 }
 
-__global__ void tree_clear(uint32_t* p, uint32_t n) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+__global__ void tree_clear(uint32_t* visit, uint32_t n, TreeWs ws) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) { *ws.error = 0; *ws.ndef = 0; *ws.vmax = 0; }
+    for (uint32_t i = t; i < n; i += gridDim.x * blockDim.x) visit[i] = 0;
 }
 
-__global__ __launch_bounds__(256) void leaf_climb(const kr_bvh_node* __restrict__ nodes,
-                                                  const uint32_t* __restrict__ parents,
-                                                  const kr_vec3* __restrict__ V,
-                                                  const kr_uvec4* __restrict__ F, kr_obb3* obbs,
-                                                  kr_mat4* mats, TreeWs ws, uint32_t N) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    const uint32_t I = N - 1, total = 2 * N - 1;
-    uint32_t cur = I + i;
-    const uint32_t f = nodes[cur].u.primitivesOffset;
-    if (f >= N) { atomicOr(ws.error, 4u); return; }  // faces has one entry per leaf
-    const kr_uvec4 face = F[f];
-    const f3 v[3] = {ldv(V, face.x), ldv(V, face.y), ldv(V, face.z)};
-    const uint32_t vid[3] = {face.x, face.y, face.z};
+// A node is complete when both children arrived: phase-1 arrivals add an odd
+// owner tag, phase-2 arrivals add 1, so two arrivals always sum to an even,
+// non-zero count and one arrival to an odd one.
+__device__ __forceinline__ bool complete(uint32_t v) { return v != 0 && (v & 1u) == 0; }
 
-    Kdop k;
-    {
-        float pr[7];
-        slab7(v[0], pr);
+__device__ __forceinline__ void merge_left_first(const Kdop& L, const Kdop& R, Kdop& m) {
+    m = L;
 #pragma unroll
-        for (int s = 0; s < 7; ++s) { k.mn[s] = k.mx[s] = pr[s]; k.imn[s] = k.imx[s] = vid[0]; }
+    for (int s = 0; s < 7; ++s) {
+        if (R.mn[s] < m.mn[s]) { m.mn[s] = R.mn[s]; m.imn[s] = R.imn[s]; }
+        if (R.mx[s] > m.mx[s]) { m.mx[s] = R.mx[s]; m.imx[s] = R.imx[s]; }
+    }
+}
+
+__device__ __forceinline__ uint32_t wg_fetch_add(uint32_t* p, uint32_t v) {
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Phase 1: workgroup b owns leaves [b*chunk, (b+1)*chunk) and processes them
+// in rounds of blockDim.x.  At a parent, the first child to arrive stores its
+// odd owner tag (2b+1) and stops.  The second continues only if the tag is its
+// own: both children were then completed inside this workgroup, so
+// workgroup-scope acq_rel ordering makes the sibling's k-DOP visible and no
+// device-scope fence is needed.  A node whose children came from different
+// workgroups is deferred to phase 2, which runs after a kernel boundary.
+__global__ __launch_bounds__(1024) void climb_local(const kr_bvh_node* __restrict__ nodes,
+                                                    const uint32_t* __restrict__ parents,
+                                                    const kr_vec3* __restrict__ V,
+                                                    const kr_uvec4* __restrict__ F, kr_obb3* obbs,
+                                                    kr_mat4* mats, TreeWs ws, uint32_t N, uint32_t chunk) {
+    const uint32_t I = N - 1, total = 2 * N - 1;
+    const uint32_t tag = 2u * blockIdx.x + 1u;
+    uint32_t vmax = 0;
+    for (uint32_t r = threadIdx.x; r < chunk; r += blockDim.x) {
+        const uint64_t i64 = (uint64_t)blockIdx.x * chunk + r;
+        if (i64 >= N) break;
+        const uint32_t i = (uint32_t)i64;
+        uint32_t cur = I + i;
+        const uint32_t f = nodes[cur].u.primitivesOffset;
+        if (f >= N) { atomicOr(ws.error, 4u); continue; }  // faces has one entry per leaf
+        const kr_uvec4 face = F[f];
+        vmax = max(vmax, max(face.x, max(face.y, face.z)));
+        const f3 v[3] = {ldv(V, face.x), ldv(V, face.y), ldv(V, face.z)};
+        const uint32_t vid[3] = {face.x, face.y, face.z};
+        Kdop k;
+        {
+            float pr[7];
+            slab7(v[0], pr);
 #pragma unroll
-        for (int j = 1; j < 3; ++j) {
-            slab7(v[j], pr);
+            for (int s = 0; s < 7; ++s) { k.mn[s] = k.mx[s] = pr[s]; k.imn[s] = k.imx[s] = vid[0]; }
 #pragma unroll
-            for (int s = 0; s < 7; ++s) {
-                if (pr[s] < k.mn[s]) { k.mn[s] = pr[s]; k.imn[s] = vid[j]; }
-                if (pr[s] > k.mx[s]) { k.mx[s] = pr[s]; k.imx[s] = vid[j]; }
+            for (int j = 1; j < 3; ++j) {
+                slab7(v[j], pr);
+#pragma unroll
+                for (int s = 0; s < 7; ++s) {
+                    if (pr[s] < k.mn[s]) { k.mn[s] = pr[s]; k.imn[s] = vid[j]; }
+                    if (pr[s] > k.mx[s]) { k.mx[s] = pr[s]; k.imx[s] = vid[j]; }
+                }
             }
         }
-    }
-    kdop_store(ws.kdop, cur, k);
-    leaf_obb(v[0], v[1], v[2], &obbs[cur], &mats[cur]);
+        kdop_store(ws.kdop, cur, k);
+        leaf_obb(v[0], v[1], v[2], &obbs[cur], &mats[cur]);
 
-    uint32_t p = parents[cur];
-    for (uint32_t step = 0; p != kInvalid; ++step) {
-        if (p >= I || step >= N) { atomicOr(ws.error, 1u); return; }
-        // release: this lane's k-DOP of `cur` is visible before the count
-        uint32_t old = atomic_add_acqrel(&ws.visit[p], 1u);
-        if (old == 0) return;  // sibling not done yet; it will continue
-        const uint32_t l = nodes[p].u.children.left, r = nodes[p].u.children.right;
-        const bool am_left = (l == cur);
-        const uint32_t sib = am_left ? r : l;
-        if (sib >= total || (!am_left && r != cur)) { atomicOr(ws.error, 2u); return; }
-        Kdop o;
-        kdop_load(ws.kdop, sib, o);
-        const Kdop& L = am_left ? k : o;
-        const Kdop& R = am_left ? o : k;
-        Kdop m = L;
-#pragma unroll
-        for (int s = 0; s < 7; ++s) {
-            if (R.mn[s] < m.mn[s]) { m.mn[s] = R.mn[s]; m.imn[s] = R.imn[s]; }
-            if (R.mx[s] > m.mx[s]) { m.mx[s] = R.mx[s]; m.imx[s] = R.imx[s]; }
+        uint32_t p = parents[cur];
+        for (uint32_t step = 0; p != kInvalid; ++step) {
+            if (p >= I || step >= N) { atomicOr(ws.error, 1u); break; }
+            const uint32_t old = wg_fetch_add(&ws.visit[p], tag);
+            if (old == 0) break;  // first arrival: the sibling continues
+            if (old != tag) {     // sibling finished in another workgroup
+                const uint32_t slot = atomicAdd(ws.ndef, 1u);
+                if (slot < I) ws.deferred[slot] = p;
+                break;
+            }
+            const uint32_t l = nodes[p].u.children.left, rr = nodes[p].u.children.right;
+            const bool am_left = (l == cur);
+            const uint32_t sib = am_left ? rr : l;
+            if (sib >= total || (!am_left && rr != cur)) { atomicOr(ws.error, 2u); break; }
+            Kdop o;
+            kdop_load(ws.kdop, sib, o);
+            Kdop m;
+            if (am_left) merge_left_first(k, o, m); else merge_left_first(o, k, m);
+            k = m;
+            kdop_store(ws.kdop, p, k);
+            cur = p;
+            p = parents[cur];
         }
-        k = m;
-        kdop_store(ws.kdop, p, k);
-        cur = p;
-        p = parents[cur];
+    }
+#pragma unroll
+    for (int m = 1; m < kWave; m <<= 1) vmax = max(vmax, (uint32_t)__shfl_xor((int)vmax, m, kWave));
+    if (lane_id() == 0) atomicMax(ws.vmax, vmax);
+}
+
+// Phase 2: one workgroup completes the deferred nodes and everything above
+// them.  Data written in phase 1 is visible after the kernel boundary; data
+// written here is exchanged with workgroup-scope acq_rel (single workgroup).
+__global__ __launch_bounds__(1024) void climb_top(const kr_bvh_node* __restrict__ nodes,
+                                                  const uint32_t* __restrict__ parents, TreeWs ws,
+                                                  uint32_t N) {
+    const uint32_t I = N - 1, total = 2 * N - 1;
+    const uint32_t nd = min(*ws.ndef, I);
+    for (uint32_t d = threadIdx.x; d < nd; d += blockDim.x) {
+        uint32_t cur = ws.deferred[d];
+        if (cur >= I) { atomicOr(ws.error, 1u); continue; }
+        const uint32_t l0 = nodes[cur].u.children.left, r0 = nodes[cur].u.children.right;
+        if (l0 >= total || r0 >= total) { atomicOr(ws.error, 2u); continue; }
+        Kdop a, b, k;
+        kdop_load(ws.kdop, l0, a);
+        kdop_load(ws.kdop, r0, b);
+        merge_left_first(a, b, k);
+        kdop_store(ws.kdop, cur, k);
+        uint32_t p = parents[cur];
+        for (uint32_t step = 0; p != kInvalid; ++step) {
+            if (p >= I || step >= N) { atomicOr(ws.error, 1u); break; }
+            const uint32_t old = wg_fetch_add(&ws.visit[p], 1u);
+            if (old == 0) break;
+            if ((old & 1u) == 0) { atomicOr(ws.error, 2u); break; }  // a third arrival
+            const uint32_t l = nodes[p].u.children.left, rr = nodes[p].u.children.right;
+            const bool am_left = (l == cur);
+            const uint32_t sib = am_left ? rr : l;
+            if (sib >= total || (!am_left && rr != cur)) { atomicOr(ws.error, 2u); break; }
+            Kdop o, m;
+            kdop_load(ws.kdop, sib, o);
+            if (am_left) merge_left_first(k, o, m); else merge_left_first(o, k, m);
+            k = m;
+            kdop_store(ws.kdop, p, k);
+            cur = p;
+            p = parents[cur];
+        }
     }
 }
 
@@ -288,9 +361,9 @@ __device__ __forceinline__ void from_normal_edges(const f3* P, f3 n, f3 e0, f3 e
 __global__ __launch_bounds__(256) void candidates(const kr_vec3* __restrict__ V, TreeWs ws, uint32_t I) {
     const uint32_t node = blockIdx.x * blockDim.x + threadIdx.x;
     if (node >= I) return;
-    if (*ws.error != 0 || ws.visit[node] != 2) {
+    if (*ws.error != 0 || !complete(ws.visit[node])) {
         // malformed tree: never read the ids of a node that did not complete
-        if (ws.visit[node] != 2) atomicOr(ws.error, 8u);
+        if (!complete(ws.visit[node])) atomicOr(ws.error, 8u);
         Cand c = {};
         c.degen = 1;
         ws.cand[node] = c;
@@ -300,10 +373,11 @@ __global__ __launch_bounds__(256) void candidates(const kr_vec3* __restrict__ V,
     Kdop k;
     kdop_load(ws.kdop, node, k);
     f3 P[14];
+    const uint32_t vmax = *ws.vmax;  // ids never exceed the largest face vertex id
 #pragma unroll
     for (int s = 0; s < 7; ++s) {
-        P[2 * s] = ldv(V, k.imn[s]);
-        P[2 * s + 1] = ldv(V, k.imx[s]);
+        P[2 * s] = ldv(V, min(k.imn[s], vmax));
+        P[2 * s + 1] = ldv(V, min(k.imx[s], vmax));
     }
     const f3 alen = f3{k.mx[0] - k.mn[0], k.mx[1] - k.mn[1], k.mx[2] - k.mn[2]};
     Best b;
@@ -426,7 +500,7 @@ __global__ __launch_bounds__(256) void refit(const kr_bvh_node* __restrict__ nod
 __global__ __launch_bounds__(256) void finalize(TreeWs ws, kr_obb3* obbs, kr_mat4* mats, uint32_t I) {
     const uint32_t node = blockIdx.x * blockDim.x + threadIdx.x;
     if (node >= I) return;
-    if (*ws.error != 0 || ws.visit[node] != 2) {
+    if (*ws.error != 0 || !complete(ws.visit[node])) {
         const float nan = __builtin_nanf("");
         store_obb(&obbs[node], f3{nan, nan, nan}, f3{nan, nan, nan}, f3{nan, nan, nan}, f3{nan, nan, nan},
                   f3{nan, nan, nan});
@@ -473,7 +547,7 @@ __global__ void copy_ids(TreeWs ws, uint32_t* ids, uint32_t total) {
     const uint32_t node = blockIdx.x * blockDim.x + threadIdx.x;
     if (node >= total) return;
     const uint32_t I = (total - 1) / 2;
-    if (*ws.error != 0 || (node < I && ws.visit[node] != 2)) {
+    if (*ws.error != 0 || (node < I && !complete(ws.visit[node]))) {
 #pragma unroll
         for (int s = 0; s < 14; ++s) ids[(size_t)node * 14 + s] = 0xFFFFFFFFu;
         return;
@@ -493,7 +567,10 @@ size_t tree_carve(TreeWs* ws, void* base, size_t cap, uint32_t N) {
     Carver c{(char*)base, 0, cap};
     const size_t I = N ? N - 1 : 0, total = N ? 2 * (size_t)N - 1 : 0;
     ws->visit = c.take<uint32_t>(I + 1);
-    ws->error = c.take<uint32_t>(1);
+    ws->error = c.take<uint32_t>(4);
+    ws->ndef = ws->error ? ws->error + 1 : nullptr;
+    ws->vmax = ws->error ? ws->error + 2 : nullptr;
+    ws->deferred = c.take<uint32_t>(I + 1);
     ws->kdop = c.take<float4>(total * 7);
     ws->cand = c.take<Cand>(I + 1);
     ws->range = c.take<int32_t>((I + 1) * 6);
@@ -509,9 +586,12 @@ kr_error tree_launch(const kr_bvh_node* nodes, const uint32_t* parents, const kr
     if (workspace == nullptr || ws_bytes < need) return "obvh_obb_tree: workspace too small";
     const uint32_t I = N - 1;
     auto blocks = [](uint32_t n, uint32_t b) { return dim3((n + b - 1) / b); };
-    hipLaunchKernelGGL(tree_clear, dim3(1024), dim3(256), 0, stream, ws.visit, I + 2);
-    hipLaunchKernelGGL(leaf_climb, blocks(N, 256), dim3(256), 0, stream, nodes, parents, V, F, obbs, mats, ws, N);
+    hipLaunchKernelGGL(tree_clear, dim3(1024), dim3(256), 0, stream, ws.visit, I + 1, ws);
+    const uint32_t chunk = kClimbChunk;
+    hipLaunchKernelGGL(climb_local, blocks(N, chunk), dim3(1024), 0, stream, nodes, parents, V, F, obbs, mats, ws,
+                       N, chunk);
     if (I > 0) {
+        hipLaunchKernelGGL(climb_top, dim3(1), dim3(1024), 0, stream, nodes, parents, ws, N);
         hipLaunchKernelGGL(candidates, blocks(I, 256), dim3(256), 0, stream, V, ws, I);
         hipLaunchKernelGGL(refit, blocks(N, 256), dim3(256), 0, stream, nodes, parents, V, F, ws, N);
         hipLaunchKernelGGL(finalize, blocks(I, 256), dim3(256), 0, stream, ws, obbs, mats, I);

@@ -14,7 +14,10 @@ struct Cand {
 };
 
 struct TreeWs {
-    uint32_t* visit;   // internal_count (+1) arrival counters
+    uint32_t* visit;   // per internal node: sum of arrival tags (odd = one arrival)
+    uint32_t* ndef;    // number of deferred nodes
+    uint32_t* vmax;    // largest vertex id referenced by a face
+    uint32_t* deferred;  // nodes whose children finished in different workgroups
     uint32_t* error;   // 1: parent index out of range / too deep, 2: parent/child mismatch,
                        // 4: face id out of range, 8: a node did not complete
     float4* kdop;      // 7 per node: {min, min vertex id, max, max vertex id}

@@ -36,7 +36,7 @@ def lib():
         L.oracle_dito14.argtypes = [vp, C.c_int, vp, vp]
         L.oracle_points_obb.argtypes = [vp, C.c_int, vp, vp]
         L.oracle_points_obb_batch.argtypes = [vp, vp, C.c_int, vp, vp]
-        L.oracle_obb_tree.argtypes = [vp, vp, vp, vp, C.c_uint32, vp, vp, vp, vp]
+        L.oracle_obb_tree.argtypes = [vp, vp, vp, vp, C.c_uint32, vp, vp, vp, vp, vp]
         L.oracle_obb_tree.restype = C.c_int
         L.oracle_lbvh_build.argtypes = [vp, vp, C.c_uint32, vp, vp, vp, vp, vp]
         L.oracle_lbvh_build.restype = C.c_int
@@ -100,8 +100,10 @@ def lbvh_build(vertices, faces):
     return nodes, parents, counts, costs, mortons
 
 
-def obb_tree(nodes, parents, vertices, faces, with_ids: bool = False, with_ranges: bool = False):
-    """The transform method.  Returns obbs (2N-1, 15), mats (2N-1, 16) [, ids (2N-1, 14)] [, ranges]."""
+def obb_tree(nodes, parents, vertices, faces, with_ids: bool = False, with_ranges: bool = False,
+             with_axes: bool = False):
+    """The transform method.  Returns obbs (2N-1, 15), mats (2N-1, 16)
+    [, ids (2N-1, 14)] [, ranges (N-1, 6)] [, candidate axes (N-1, 10)]."""
     V = _f32(vertices).reshape(-1, 3)
     F = np.ascontiguousarray(faces, dtype=np.uint32).reshape(-1, 4)
     nodes = np.ascontiguousarray(nodes, dtype=NODE_DTYPE)
@@ -112,9 +114,11 @@ def obb_tree(nodes, parents, vertices, faces, with_ids: bool = False, with_range
     mats = np.zeros((total, 16), dtype=np.float32)
     ids = np.zeros((total, 14), dtype=np.uint32) if with_ids else None
     rr = np.zeros((max(N - 1, 1), 6), dtype=np.float32) if with_ranges else None
+    ca = np.zeros((max(N - 1, 1), 10), dtype=np.float32) if with_axes else None
     rc = lib().oracle_obb_tree(
         _p(nodes), _p(parents), _p(V), _p(F), N, _p(obbs), _p(mats),
-        _p(ids) if ids is not None else None, _p(rr) if rr is not None else None)
+        _p(ids) if ids is not None else None, _p(rr) if rr is not None else None,
+        _p(ca) if ca is not None else None)
     if rc != 0:
         raise ValueError("oracle_obb_tree: malformed tree")
     res = [obbs, mats]
@@ -122,6 +126,8 @@ def obb_tree(nodes, parents, vertices, faces, with_ids: bool = False, with_range
         res.append(ids)
     if with_ranges:
         res.append(rr)
+    if with_axes:
+        res.append(ca)
     return tuple(res)
 
 

@@ -59,12 +59,14 @@ void oracle_points_obb_batch(const float* xyz, const uint64_t* off, int nsets,
 /* The GPU transform method.  nodes/parents: 2N-1 entries (N = nprim);
  * outputs obbs/mats for every node.  kdop_ids (nullable): 14 ids per node
  * [min0..min6, max0..max6].  refit_ranges (nullable): 6 floats per node
- * (min b0..b2, max b0..b2 after refit; internal nodes only).
+ * (min b0..b2, max b0..b2 after refit; internal nodes only).  cand_axes
+ * (nullable): 10 floats per internal node, the candidate axes b0, b1, b2
+ * and 1.0 if the node is degenerate (its box is its AABB).
  * Returns 0 on success, -1 on a malformed tree. */
 int oracle_obb_tree(const onode* nodes, const uint32_t* parents,
                     const float* vertices, const uint32_t* faces /* uvec4 */,
                     uint32_t nprim, oobb* obbs, omat4* mats,
-                    uint32_t* kdop_ids, float* refit_ranges);
+                    uint32_t* kdop_ids, float* refit_ranges, float* cand_axes);
 
 /* LBVH build.  vertices AoS xyz, faces uvec4.  Outputs 2N-1 nodes, parents,
  * primitive_counts, costs; mortons_sorted (N entries, nullable). */

@@ -366,7 +366,7 @@ static int check_tree(const onode* nodes, const uint32_t* parents, uint32_t N) {
 
 int oracle_obb_tree(const onode* nodes, const uint32_t* parents, const float* V,
                     const uint32_t* faces, uint32_t N, oobb* obbs, omat4* mats,
-                    uint32_t* kdop_ids, float* refit_ranges) {
+                    uint32_t* kdop_ids, float* refit_ranges, float* cand_axes) {
     if (N == 0) return 0;
     uint32_t I = N - 1, total = 2 * N - 1;
     if (check_tree(nodes, parents, N) != 0) return -1;
@@ -424,6 +424,13 @@ int oracle_obb_tree(const onode* nodes, const uint32_t* parents, const float* V,
                     rmin.z = kmin(rmin.z, d2); rmax.z = kmax(rmax.z, d2);
                 }
             }
+        }
+        if (cand_axes) {
+            float* ca = &cand_axes[10 * v];
+            ca[0] = c->b0.x; ca[1] = c->b0.y; ca[2] = c->b0.z;
+            ca[3] = c->b1.x; ca[4] = c->b1.y; ca[5] = c->b1.z;
+            ca[6] = c->b2.x; ca[7] = c->b2.y; ca[8] = c->b2.z;
+            ca[9] = (float)c->degenerate;
         }
         if (refit_ranges) {
             float* rr = &refit_ranges[6 * v];

@@ -201,3 +201,57 @@ def test_full_size_transform_properties(torch, oracle_lib):
     mism = int((~np.all(go == eo, axis=1)).sum())
     assert mism == 0, mism
     assert np.array_equal(m1.cpu().numpy(), em)
+
+
+def random_tree(n_leaves: int, seed: int):
+    """Random binary tree over shuffled leaves: internal nodes numbered in
+    breadth-first order (root 0), leaves I..2N-2 holding a random face
+    permutation.  Sibling leaves are rarely adjacent, so the bottom-up
+    kernels exchange most k-DOPs across workgroups."""
+    rng = np.random.default_rng(seed)
+    I = n_leaves - 1
+    nodes = np.zeros(2 * n_leaves - 1, dtype=NODE_DTYPE)
+    parents = np.zeros(2 * n_leaves - 1, dtype=np.uint32)
+    parents[0] = 0xFFFFFFFF
+    leaf_slots = rng.permutation(n_leaves)  # leaf slot of each face
+    queue = [(0, np.arange(n_leaves))]
+    next_internal = 1
+    head = 0
+    while head < len(queue):
+        idx, faces = queue[head]
+        head += 1
+        cut = int(rng.integers(1, len(faces)))
+        kids = []
+        for part in (faces[:cut], faces[cut:]):
+            if len(part) == 1:
+                kids.append(I + int(leaf_slots[part[0]]))
+            else:
+                kids.append(next_internal)
+                queue.append((next_internal, part))
+                next_internal += 1
+        nodes[idx]["a"], nodes[idx]["b"] = kids
+        for k in kids:
+            parents[k] = idx
+    for f in range(n_leaves):
+        L = nodes[I + leaf_slots[f]]
+        L["a"] = f
+        L["nprim"] = 1
+    return nodes, parents
+
+
+@pytest.mark.parametrize("n", [5000, 20000])
+def test_obb_tree_random_topology(torch, oracle_lib, n):
+    say(f"[obb_tree] random topology n={n}")
+    V, F = synth.hairball(tubes=n // 40 + 1, seed=6)
+    F = F[:n]
+    nodes, parents = random_tree(n, seed=n)
+    Vd = torch.from_numpy(V).cuda()
+    Fd = torch.from_numpy(np.ascontiguousarray(F).view(np.int32)).cuda()
+    nd = torch.from_numpy(nodes.view(np.uint8)).cuda()
+    pd = torch.from_numpy(parents.view(np.int32)).cuda()
+    obbs, mats, ids = ops.obb_tree(nd, pd, Vd, Fd, with_ids=True)
+    torch.cuda.synchronize()
+    eo, em, ei = oracle_lib.obb_tree(nodes, parents, V, F, with_ids=True)
+    assert np.array_equal(ids.cpu().numpy().view(np.uint32), ei)
+    assert np.array_equal(obbs.cpu().numpy(), eo)
+    assert np.array_equal(mats.cpu().numpy(), em)
