@@ -34,6 +34,114 @@ def b_alg_transform(N: int, V: int) -> int:
     return (2 * N - 1) * (36 + 4 + 60 + 64) + 16 * N + 12 * V
 
 
+def dito_legs(args) -> dict:
+    """Batched DiTO over log-uniform sets (64 .. 2^20 points), the same sets fitted
+    one call per set, and one 131 072-point fit (device resident and legacy)."""
+    import numpy as np
+    import torch
+
+    from obvh_amd import batch, ops, synth
+
+    out = {}
+    sizes = synth.batch_sizes(args.sets, lo=64, hi=1 << 20, seed=4)
+    t0 = time.perf_counter()
+    x, y, z, off = batch.device_sets(sizes, seed=5)
+    torch.cuda.synchronize()
+    total = int(sizes.sum())
+    print(f"[bench] dito batch: {args.sets} sets, {total} points generated in {time.perf_counter() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    tuner = batch.Autotuner()
+    entry = tuner.lookup(sizes)
+    if entry is None:
+        entry, _ = tuner.tune(x, y, z, off, sizes, reps=2)
+    g = batch.BatchGraph(x, y, z, off, entry["tile_points"], entry["workers"])
+    for _ in range(3):
+        g.replay()
+    torch.cuda.synchronize()
+    reps = 20
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    w0 = time.perf_counter()
+    e0.record()
+    for _ in range(reps):
+        g.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - w0) / reps
+    dev = e0.elapsed_time(e1) / reps / 1e3
+    batched = g.out.clone()
+    out["batched"] = {
+        "sets": args.sets, "points": total, "tile_points": entry["tile_points"], "workers": entry["workers"],
+        "ms": round(dev * 1e3, 4), "wall_ms": round(wall * 1e3, 4),
+        "Gpts_per_s": round(total / dev / 1e9, 2), "sets_per_s": round(args.sets / dev, 1),
+        "achieved_GBs": round(24.0 * total / dev / 1e9, 1),
+        "roofline_frac": round(24.0 * total / dev / 1e9 / HBM_PEAK_GBS, 4),
+    }
+    print(f"[bench] dito batched: {dev*1e3:.3f} ms, {total/dev/1e9:.1f} Gpts/s", file=sys.stderr, flush=True)
+    offs = off.cpu().numpy()
+    if args.no_plain:
+        return out
+    # plain per-request launches: one obvh_dito_soa call per set
+    ws = torch.empty(ops.lib().obvh_dito_workspace_size(int(sizes.max())), dtype=torch.uint8, device="cuda")
+    res = []
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    for s in range(args.sets):
+        a, b = int(offs[s]), int(offs[s + 1])
+        res.append(ops.dito(x[a:b], y[a:b], z[a:b], workspace=ws))
+    torch.cuda.synchronize()
+    plain = time.perf_counter() - w0
+    same = bool(torch.equal(torch.stack(res), batched))
+    out["plain_per_set"] = {"s": round(plain, 4), "sets_per_s": round(args.sets / plain, 1),
+                            "Gpts_per_s": round(total / plain / 1e9, 3), "equal_to_batched": same}
+    out["batched_over_plain"] = round((args.sets / dev) / (args.sets / plain), 2)
+    print(f"[bench] dito plain per set: {plain*1e3:.1f} ms ({out['batched_over_plain']}x slower), equal={same}",
+          file=sys.stderr, flush=True)
+    # one 131 072-point set
+    p = synth.gaussian_points(131072, seed=1)
+    xs, ys, zs = (torch.from_numpy(np.ascontiguousarray(p[:, k])).cuda() for k in range(3))
+    ws1 = torch.empty(ops.lib().obvh_dito_workspace_size(131072), dtype=torch.uint8, device="cuda")
+    o1 = torch.empty(15, dtype=torch.float32, device="cuda")
+    for _ in range(5):
+        ops.dito(xs, ys, zs, workspace=ws1)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(50):
+        ops.dito(xs, ys, zs, workspace=ws1)
+    e1.record()
+    torch.cuda.synchronize()
+    single = e0.elapsed_time(e1) / 50 / 1e3
+    legacy = []
+    for _ in range(5):
+        w0 = time.perf_counter()
+        r = ops.points_obb(p)
+        legacy.append(time.perf_counter() - w0)
+    out["single_131072"] = {"device_us": round(single * 1e6, 2), "Gpts_per_s": round(131072 / single / 1e9, 3),
+                            "legacy_wall_us": round(min(legacy) * 1e6, 1)}
+    print(f"[bench] dito single 131072: {single*1e6:.1f} us device, legacy {min(legacy)*1e6:.0f} us wall",
+          file=sys.stderr, flush=True)
+    del o1, r
+    if not args.no_cpu:
+        import oracle
+
+        # CPU DiTO on a bounded sample of the batch (sets 0..K until ~10 s of work)
+        xh, yh, zh = x.cpu().numpy(), y.cpu().numpy(), z.cpu().numpy()
+        done, pts, t_cpu = 0, 0, 0.0
+        for s in range(args.sets):
+            a, b = int(offs[s]), int(offs[s + 1])
+            P = np.stack([xh[a:b], yh[a:b], zh[a:b]], 1)
+            w0 = time.perf_counter()
+            oracle.points_obb(P)
+            t_cpu += time.perf_counter() - w0
+            done += 1
+            pts += b - a
+            if t_cpu > 8.0:
+                break
+        out["cpu_baseline"] = {"Gpts_per_s": round(pts / t_cpu / 1e9, 4), "sets_per_s": round(done / t_cpu, 1),
+                               "cores": 1, "kind": "port",
+                               "sample": f"first {done} sets of the batch ({pts} points, {t_cpu:.1f} s)"}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -41,6 +149,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--freq", type=int, default=224, help="icosphere frequency (224 -> 1 003 520 triangles)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-dito", action="store_true", help="skip the DiTO legs")
+    ap.add_argument("--sets", type=int, default=8192, help="batched DiTO: number of sets")
+    ap.add_argument("--no-plain", action="store_true", help="skip the one-call-per-set DiTO comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs the OBB path on a single MI355X "
@@ -113,6 +224,14 @@ def main() -> int:
         }
         print(f"[bench] cpu oracle {cpu_s:.2f}s, parity={parity}", file=sys.stderr, flush=True)
 
+    dito = None
+    if not args.no_dito:
+        try:
+            dito = dito_legs(args)
+        except Exception as e:  # the transform line must still be reported
+            print(f"[bench] DiTO legs failed: {e!r}", file=sys.stderr, flush=True)
+            dito = {"error": repr(e)}
+
     line = {
         "metric": "OBB-tree transform Mnodes/s; batched DiTO Gpts/s; served p50/p99 ms",
         "value": round(value, 2),
@@ -147,6 +266,7 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
         "parity": parity,
+        "dito": dito,
     }
     print(json.dumps(line))
     return 0

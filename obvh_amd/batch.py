@@ -1,0 +1,151 @@
+"""Batched DiTO: many uneven point sets in one pass pair, tuned and graph-captured.
+
+* ``device_sets``   builds a batch of Gaussian sets directly in HBM (SoA).
+* ``Autotuner``     times (tile_points, workers) candidates on the real batch
+                    with HIP events and caches the winner per shape bucket in a
+                    JSON file (``$OBVH_CACHE_DIR`` or ``~/.cache/obvh_amd``).
+* ``BatchGraph``    captures one ``obvh_dito_batch`` call (setup, tile table,
+                    pass 1, candidates, pass 2, finalise: six kernels, no
+                    allocation, no memset) into a HIP graph and replays it.
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+import time
+
+import numpy as np
+
+from . import ops, synth
+from ._lib import lib
+
+TILE_CANDIDATES = (2048, 4096, 8192, 16384, 32768, 65536)
+WORKER_CANDIDATES = (1024, 2048, 4096, 8192)
+
+
+def device_sets(sizes, seed: int = 5, device: str = "cuda"):
+    """Sets of N(0, diag(1, r1, r2)^2) (r ~ U(1, 20), sorted descending) under a
+    random rotation, generated on the GPU.  Returns (x, y, z, offsets int64)."""
+    import torch
+
+    sizes = np.asarray(sizes, dtype=np.int64)
+    rng = np.random.default_rng(seed)
+    ratios = rng.uniform(1.0, 20.0, size=(sizes.size, 2))
+    R = synth.random_rotation(rng, sizes.size)
+    off = np.zeros(sizes.size + 1, dtype=np.int64)
+    off[1:] = np.cumsum(sizes)
+    total = int(off[-1])
+    X = torch.empty(total, dtype=torch.float32, device=device)
+    Y = torch.empty_like(X)
+    Z = torch.empty_like(X)
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    Rt = torch.from_numpy(R.astype(np.float32)).to(device)
+    for s, n in enumerate(sizes):
+        n = int(n)
+        if n == 0:
+            continue
+        std = np.sort(np.array([1.0, ratios[s, 0], ratios[s, 1]]))[::-1].astype(np.float32)
+        p = torch.randn((n, 3), generator=g, device=device) * torch.from_numpy(std.copy()).to(device)
+        p = p @ Rt[s].T
+        a, b = int(off[s]), int(off[s + 1])
+        X[a:b] = p[:, 0]
+        Y[a:b] = p[:, 1]
+        Z[a:b] = p[:, 2]
+    return X, Y, Z, torch.from_numpy(off).to(device)
+
+
+def shape_key(sizes) -> str:
+    """Bucket of a batch shape: set count, total points and the size spread."""
+    s = np.asarray(sizes, dtype=np.int64)
+    if s.size == 0:
+        return "empty"
+    q = np.quantile(s, [0.1, 0.5, 0.9])
+    b = lambda v: int(round(math.log2(max(float(v), 1.0))))  # noqa: E731
+    return f"n{b(s.size)}_t{b(s.sum())}_q{b(q[0])}-{b(q[1])}-{b(q[2])}"
+
+
+class Autotuner:
+    """Picks (tile_points, workers) for obvh_dito_batch by measurement."""
+
+    def __init__(self, path: str | None = None):
+        d = os.environ.get("OBVH_CACHE_DIR", os.path.join(os.path.expanduser("~"), ".cache", "obvh_amd"))
+        self.path = path or os.path.join(d, "dito_batch_tune.json")
+        self.table: dict[str, dict] = {}
+        if os.path.exists(self.path):
+            try:
+                with open(self.path) as f:
+                    self.table = json.load(f)
+            except (OSError, ValueError):
+                self.table = {}
+
+    def _save(self) -> None:
+        try:
+            os.makedirs(os.path.dirname(self.path), exist_ok=True)
+            with open(self.path, "w") as f:
+                json.dump(self.table, f, indent=1, sort_keys=True)
+        except OSError:
+            pass
+
+    def lookup(self, sizes):
+        return self.table.get(shape_key(sizes))
+
+    def tune(self, x, y, z, offsets, sizes, reps: int = 3, tiles=TILE_CANDIDATES, workers=WORKER_CANDIDATES,
+             verbose: bool = False):
+        """Time every candidate on this batch; store and return the best."""
+        import torch
+
+        nsets = int(offsets.numel()) - 1
+        total = int(x.numel())
+        best = None
+        results = []
+        for t in tiles:
+            ws = torch.empty(lib().obvh_dito_batch_workspace_size(nsets, total, t), dtype=torch.uint8, device="cuda")
+            for w in workers:
+                ops.dito_batch(x, y, z, offsets, tile_points=t, workers=w, workspace=ws)  # warm
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                for _ in range(reps):
+                    ops.dito_batch(x, y, z, offsets, tile_points=t, workers=w, workspace=ws)
+                e1.record()
+                torch.cuda.synchronize()
+                ms = e0.elapsed_time(e1) / reps
+                results.append((ms, t, w))
+                if verbose:
+                    print(f"[autotune] tile={t} workers={w}: {ms:.3f} ms", flush=True)
+                if best is None or ms < best[0]:
+                    best = (ms, t, w)
+        entry = {"tile_points": best[1], "workers": best[2], "ms": round(best[0], 4),
+                 "tuned_at": time.strftime("%Y-%m-%dT%H:%M:%S")}
+        self.table[shape_key(sizes)] = entry
+        self._save()
+        return entry, sorted(results)
+
+
+class BatchGraph:
+    """One obvh_dito_batch call captured into a HIP graph (fixed buffers)."""
+
+    def __init__(self, x, y, z, offsets, tile_points: int, workers: int):
+        import torch
+
+        self.nsets = int(offsets.numel()) - 1
+        total = int(x.numel())
+        self.args = (x, y, z, offsets)
+        self.tile, self.workers = tile_points, workers
+        self.ws = torch.empty(lib().obvh_dito_batch_workspace_size(self.nsets, total, tile_points),
+                              dtype=torch.uint8, device="cuda")
+        self.out = None
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):  # warm-up outside capture
+            self.out = ops.dito_batch(x, y, z, offsets, tile_points=tile_points, workers=workers, workspace=self.ws)
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.out = ops.dito_batch(x, y, z, offsets, tile_points=tile_points, workers=workers, workspace=self.ws)
+
+    def replay(self):
+        self.graph.replay()
+        return self.out

@@ -8,7 +8,10 @@ namespace obvh {
 
 struct LbvhWs {
     int32_t* cbox;       // centroid box as order-preserving keys (min xyz, max xyz)
-    uint32_t* visit;     // arrival counters (internal nodes)
+    uint64_t* visit;     // arrival words (internal nodes): count + owner tags
+    uint32_t* nlist;     // per phase: number of deferred nodes
+    uint32_t* list;      // per phase: deferred nodes
+    uint2* span;         // per node: first/last leaf position
     uint32_t* keys_in;   // Morton codes by face
     uint32_t* keys_out;  // sorted codes
     uint32_t* vals_in;   // face ids

@@ -33,7 +33,8 @@
 namespace obvh {
 
 constexpr uint32_t kInvalid = 0xFFFFFFFFu;
-constexpr uint32_t kClimbChunk = 4096;  // leaves per phase-1 workgroup
+constexpr uint32_t kChunk0 = 1024;    // leaves per workgroup, first climb phase
+constexpr uint32_t kChunk1 = 32768;   // second phase
 
 struct Mat4 {
     float v[16];
@@ -175,16 +176,21 @@ __device__ __forceinline__ void leaf_obb(f3 v0, f3 v1, f3 v2, kr_obb3* obb, kr_m
     store_mat(mat, M);
 }
 
-__global__ void tree_clear(uint32_t* visit, uint32_t n, TreeWs ws) {
+__global__ void tree_clear(uint64_t* visit, uint32_t n, TreeWs ws) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t == 0) { *ws.error = 0; *ws.ndef = 0; *ws.vmax = 0; }
+    if (t == 0) {
+        *ws.error = 0;
+        *ws.vmax = 0;
+        for (int k = 0; k < kMaxPhases; ++k) ws.nlist[k] = 0;
+    }
     for (uint32_t i = t; i < n; i += gridDim.x * blockDim.x) visit[i] = 0;
 }
 
-// A node is complete when both children arrived: phase-1 arrivals add an odd
-// owner tag, phase-2 arrivals add 1, so two arrivals always sum to an even,
-// non-zero count and one arrival to an odd one.
-__device__ __forceinline__ bool complete(uint32_t v) { return v != 0 && (v & 1u) == 0; }
+// Arrival word of an internal node: low 32 bits count the children that
+// finished, high 32 bits sum their tags.  tag = (phase << 24) | (owner + 1)
+// identifies the kernel phase and the workgroup that completed the child.
+__device__ __forceinline__ bool complete(uint64_t v) { return (uint32_t)v == 2u; }
+__device__ __forceinline__ uint32_t make_tag(uint32_t phase, uint32_t owner) { return (phase << 24) | (owner + 1u); }
 
 __device__ __forceinline__ void merge_left_first(const Kdop& L, const Kdop& R, Kdop& m) {
     m = L;
@@ -195,77 +201,20 @@ __device__ __forceinline__ void merge_left_first(const Kdop& L, const Kdop& R, K
     }
 }
 
-__device__ __forceinline__ uint32_t wg_fetch_add(uint32_t* p, uint32_t v) {
-    return __hip_atomic_fetch_add(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-
-// Phase 1: workgroup b owns leaves [b*chunk, (b+1)*chunk) and processes them
-// in rounds of blockDim.x.  At a parent, the first child to arrive stores its
-// odd owner tag (2b+1) and stops.  The second continues only if the tag is its
-// own: both children were then completed inside this workgroup, so
-// workgroup-scope acq_rel ordering makes the sibling's k-DOP visible and no
-// device-scope fence is needed.  A node whose children came from different
-// workgroups is deferred to phase 2, which runs after a kernel boundary.
-__global__ __launch_bounds__(1024) void climb_local(const kr_bvh_node* __restrict__ nodes,
-                                                    const uint32_t* __restrict__ parents,
-                                                    const kr_vec3* __restrict__ V,
-                                                    const kr_uvec4* __restrict__ F, kr_obb3* obbs,
-                                                    kr_mat4* mats, TreeWs ws, uint32_t N, uint32_t chunk) {
-    const uint32_t I = N - 1, total = 2 * N - 1;
-    const uint32_t tag = 2u * blockIdx.x + 1u;
+// Leaf boxes (transforms.cu:803-834) and the largest referenced vertex id.
+__global__ __launch_bounds__(256) void leaf_boxes(const kr_bvh_node* __restrict__ nodes,
+                                                  const kr_vec3* __restrict__ V,
+                                                  const kr_uvec4* __restrict__ F, kr_obb3* obbs,
+                                                  kr_mat4* mats, TreeWs ws, uint32_t N) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t vmax = 0;
-    for (uint32_t r = threadIdx.x; r < chunk; r += blockDim.x) {
-        const uint64_t i64 = (uint64_t)blockIdx.x * chunk + r;
-        if (i64 >= N) break;
-        const uint32_t i = (uint32_t)i64;
-        uint32_t cur = I + i;
+    if (i < N) {
+        const uint32_t cur = N - 1 + i;
         const uint32_t f = nodes[cur].u.primitivesOffset;
-        if (f >= N) { atomicOr(ws.error, 4u); continue; }  // faces has one entry per leaf
-        const kr_uvec4 face = F[f];
-        vmax = max(vmax, max(face.x, max(face.y, face.z)));
-        const f3 v[3] = {ldv(V, face.x), ldv(V, face.y), ldv(V, face.z)};
-        const uint32_t vid[3] = {face.x, face.y, face.z};
-        Kdop k;
-        {
-            float pr[7];
-            slab7(v[0], pr);
-#pragma unroll
-            for (int s = 0; s < 7; ++s) { k.mn[s] = k.mx[s] = pr[s]; k.imn[s] = k.imx[s] = vid[0]; }
-#pragma unroll
-            for (int j = 1; j < 3; ++j) {
-                slab7(v[j], pr);
-#pragma unroll
-                for (int s = 0; s < 7; ++s) {
-                    if (pr[s] < k.mn[s]) { k.mn[s] = pr[s]; k.imn[s] = vid[j]; }
-                    if (pr[s] > k.mx[s]) { k.mx[s] = pr[s]; k.imx[s] = vid[j]; }
-                }
-            }
-        }
-        kdop_store(ws.kdop, cur, k);
-        leaf_obb(v[0], v[1], v[2], &obbs[cur], &mats[cur]);
-
-        uint32_t p = parents[cur];
-        for (uint32_t step = 0; p != kInvalid; ++step) {
-            if (p >= I || step >= N) { atomicOr(ws.error, 1u); break; }
-            const uint32_t old = wg_fetch_add(&ws.visit[p], tag);
-            if (old == 0) break;  // first arrival: the sibling continues
-            if (old != tag) {     // sibling finished in another workgroup
-                const uint32_t slot = atomicAdd(ws.ndef, 1u);
-                if (slot < I) ws.deferred[slot] = p;
-                break;
-            }
-            const uint32_t l = nodes[p].u.children.left, rr = nodes[p].u.children.right;
-            const bool am_left = (l == cur);
-            const uint32_t sib = am_left ? rr : l;
-            if (sib >= total || (!am_left && rr != cur)) { atomicOr(ws.error, 2u); break; }
-            Kdop o;
-            kdop_load(ws.kdop, sib, o);
-            Kdop m;
-            if (am_left) merge_left_first(k, o, m); else merge_left_first(o, k, m);
-            k = m;
-            kdop_store(ws.kdop, p, k);
-            cur = p;
-            p = parents[cur];
+        if (f < N) {
+            const kr_uvec4 face = F[f];
+            vmax = max(face.x, max(face.y, face.z));
+            leaf_obb(ldv(V, face.x), ldv(V, face.y), ldv(V, face.z), &obbs[cur], &mats[cur]);
         }
     }
 #pragma unroll
@@ -273,30 +222,86 @@ __global__ __launch_bounds__(1024) void climb_local(const kr_bvh_node* __restric
     if (lane_id() == 0) atomicMax(ws.vmax, vmax);
 }
 
-// Phase 2: one workgroup completes the deferred nodes and everything above
-// them.  Data written in phase 1 is visible after the kernel boundary; data
-// written here is exchanged with workgroup-scope acq_rel (single workgroup).
-__global__ __launch_bounds__(1024) void climb_top(const kr_bvh_node* __restrict__ nodes,
-                                                  const uint32_t* __restrict__ parents, TreeWs ws,
-                                                  uint32_t N) {
+// One phase of the bottom-up k-DOP pass.  Workgroup w of phase `phase` owns
+// the leaf positions [w*chunk, (w+1)*chunk).  Phase 0 starts from the leaves;
+// later phases start from the nodes the previous phase deferred (those whose
+// leaf span lies inside the workgroup's range).  At a parent, the first
+// arriving child stops; the second continues if the first was completed in
+// an earlier kernel (visible after the kernel boundary) or by its own
+// workgroup (workgroup-scope acq_rel on the arrival word orders the data);
+// otherwise the parent is deferred to the next phase.  The last phase owns
+// every leaf and never defers.  No device-scope fence is issued.
+__global__ __launch_bounds__(1024) void climb_phase(const kr_bvh_node* __restrict__ nodes,
+                                                    const uint32_t* __restrict__ parents,
+                                                    const kr_vec3* __restrict__ V,
+                                                    const kr_uvec4* __restrict__ F, TreeWs ws, uint32_t N,
+                                                    uint32_t phase, uint32_t chunk) {
     const uint32_t I = N - 1, total = 2 * N - 1;
-    const uint32_t nd = min(*ws.ndef, I);
-    for (uint32_t d = threadIdx.x; d < nd; d += blockDim.x) {
-        uint32_t cur = ws.deferred[d];
-        if (cur >= I) { atomicOr(ws.error, 1u); continue; }
-        const uint32_t l0 = nodes[cur].u.children.left, r0 = nodes[cur].u.children.right;
-        if (l0 >= total || r0 >= total) { atomicOr(ws.error, 2u); continue; }
-        Kdop a, b, k;
-        kdop_load(ws.kdop, l0, a);
-        kdop_load(ws.kdop, r0, b);
-        merge_left_first(a, b, k);
+    const uint32_t tag = make_tag(phase, blockIdx.x);
+    const uint64_t lo = (uint64_t)blockIdx.x * chunk, hi = lo + chunk;  // owned leaf positions
+    const uint32_t nin = phase == 0 ? 0 : min(ws.nlist[phase - 1], I);
+    const uint32_t* in = phase == 0 ? nullptr : ws.list + (size_t)(phase - 1) * (I + 1);
+    uint32_t* out = ws.list + (size_t)phase * (I + 1);
+    const uint32_t nstart = phase == 0 ? chunk : nin;
+    for (uint32_t r = threadIdx.x; r < nstart; r += blockDim.x) {
+        uint32_t cur;
+        Kdop k;
+        uint2 span;
+        if (phase == 0) {
+            const uint64_t pos = lo + r;
+            if (pos >= N) break;
+            cur = I + (uint32_t)pos;
+            const uint32_t f = nodes[cur].u.primitivesOffset;
+            if (f >= N) { atomicOr(ws.error, 4u); continue; }  // faces has one entry per leaf
+            const kr_uvec4 face = F[f];
+            const uint32_t vid[3] = {face.x, face.y, face.z};
+            float pr[7];
+            slab7(ldv(V, vid[0]), pr);
+#pragma unroll
+            for (int s = 0; s < 7; ++s) { k.mn[s] = k.mx[s] = pr[s]; k.imn[s] = k.imx[s] = vid[0]; }
+#pragma unroll
+            for (int j = 1; j < 3; ++j) {
+                slab7(ldv(V, vid[j]), pr);
+#pragma unroll
+                for (int s = 0; s < 7; ++s) {
+                    if (pr[s] < k.mn[s]) { k.mn[s] = pr[s]; k.imn[s] = vid[j]; }
+                    if (pr[s] > k.mx[s]) { k.mx[s] = pr[s]; k.imx[s] = vid[j]; }
+                }
+            }
+            span = make_uint2((uint32_t)pos, (uint32_t)pos);
+        } else {
+            cur = in[r];
+            if (cur >= I) { atomicOr(ws.error, 1u); continue; }
+            const uint32_t l0 = nodes[cur].u.children.left, r0 = nodes[cur].u.children.right;
+            if (l0 >= total || r0 >= total) { atomicOr(ws.error, 2u); continue; }
+            const uint2 sa = ws.span[l0], sb = ws.span[r0];
+            span = make_uint2(min(sa.x, sb.x), max(sa.y, sb.y));
+            if (span.x < lo || span.x >= hi) continue;  // another workgroup owns its first leaf
+            if (span.y >= hi) {                         // wider than this workgroup: next phase
+                const uint32_t slot = atomicAdd(&ws.nlist[phase], 1u);
+                if (slot < I) out[slot] = cur;
+                continue;
+            }
+            Kdop a, b;
+            kdop_load(ws.kdop, l0, a);
+            kdop_load(ws.kdop, r0, b);
+            merge_left_first(a, b, k);
+        }
         kdop_store(ws.kdop, cur, k);
+        ws.span[cur] = span;
         uint32_t p = parents[cur];
         for (uint32_t step = 0; p != kInvalid; ++step) {
             if (p >= I || step >= N) { atomicOr(ws.error, 1u); break; }
-            const uint32_t old = wg_fetch_add(&ws.visit[p], 1u);
-            if (old == 0) break;
-            if ((old & 1u) == 0) { atomicOr(ws.error, 2u); break; }  // a third arrival
+            const uint64_t old = __hip_atomic_fetch_add(&ws.visit[p], ((uint64_t)tag << 32) | 1ull,
+                                                        __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+            const uint32_t cnt = (uint32_t)old, t1 = (uint32_t)(old >> 32);
+            if (cnt == 0) break;  // first arrival: the sibling continues
+            if (cnt != 1) { atomicOr(ws.error, 2u); break; }
+            if (t1 != tag && (t1 >> 24) >= phase) {  // sibling done by another workgroup of this phase
+                const uint32_t slot = atomicAdd(&ws.nlist[phase], 1u);
+                if (slot < I) out[slot] = p;
+                break;
+            }
             const uint32_t l = nodes[p].u.children.left, rr = nodes[p].u.children.right;
             const bool am_left = (l == cur);
             const uint32_t sib = am_left ? rr : l;
@@ -305,7 +310,10 @@ __global__ __launch_bounds__(1024) void climb_top(const kr_bvh_node* __restrict_
             kdop_load(ws.kdop, sib, o);
             if (am_left) merge_left_first(k, o, m); else merge_left_first(o, k, m);
             k = m;
+            const uint2 so = ws.span[sib];
+            span = make_uint2(min(span.x, so.x), max(span.y, so.y));
             kdop_store(ws.kdop, p, k);
+            ws.span[p] = span;
             cur = p;
             p = parents[cur];
         }
@@ -465,33 +473,89 @@ __global__ __launch_bounds__(256) void candidates(const kr_vec3* __restrict__ V,
 }
 
 // ------------------------------------------------------------ refit --
+//
+// Exact projection ranges of every subtree vertex on each internal node's
+// candidate axes (transforms.cu:836-900).  A workgroup takes 1024
+// consecutive leaves; each lane walks from its leaf to the root and folds its
+// triangle's min/max on every ancestor's axes into an LDS hash table keyed by
+// the ancestor (LDS integer min/max on order-preserving keys).  Consecutive
+// leaves share most ancestors, so each workgroup then issues one filtered
+// global atomic per distinct ancestor instead of one per (leaf, ancestor).
+// If the table fills, the lane falls back to global atomics for that pair.
 
-__global__ __launch_bounds__(256) void refit(const kr_bvh_node* __restrict__ nodes,
-                                             const uint32_t* __restrict__ parents,
-                                             const kr_vec3* __restrict__ V,
-                                             const kr_uvec4* __restrict__ F, TreeWs ws, uint32_t N) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N || *ws.error != 0) return;
-    const uint32_t I = N - 1;
-    const uint32_t cur = I + i;
-    const kr_uvec4 face = F[nodes[cur].u.primitivesOffset];
-    const f3 v0 = ldv(V, face.x), v1 = ldv(V, face.y), v2 = ldv(V, face.z);
-    uint32_t p = parents[cur];
-    for (uint32_t step = 0; p != kInvalid && p < I && step < N; ++step) {
-        const Cand c = ws.cand[p];
-        if (!c.degen) {
-            int32_t* rk = &ws.range[(size_t)p * 6];
+constexpr int kRefitLeaves = 1024;
+constexpr int kHashSlots = 2048;  // power of two
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+
+__device__ __forceinline__ void fold_global(int32_t* rk, const int32_t v[6]) {
 #pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                const f3 b = f3{c.b[3 * a], c.b[3 * a + 1], c.b[3 * a + 2]};
-                const float d0 = dot3(v0, b), d1 = dot3(v1, b), d2 = dot3(v2, b);
-                const int32_t kmn = f2ord(kmin(kmin(d0, d1), d2));
-                const int32_t kmx = f2ord(kmax(kmax(d0, d1), d2));
-                if (kmn < rk[a]) atomicMin(&rk[a], kmn);
-                if (kmx > rk[3 + a]) atomicMax(&rk[3 + a], kmx);
+    for (int a = 0; a < 3; ++a) {
+        if (v[a] < rk[a]) atomicMin(&rk[a], v[a]);
+        if (v[3 + a] > rk[3 + a]) atomicMax(&rk[3 + a], v[3 + a]);
+    }
+}
+
+__global__ __launch_bounds__(1024) void refit(const kr_bvh_node* __restrict__ nodes,
+                                              const uint32_t* __restrict__ parents,
+                                              const kr_vec3* __restrict__ V,
+                                              const kr_uvec4* __restrict__ F, TreeWs ws, uint32_t N) {
+    __shared__ uint32_t hkey[kHashSlots];
+    __shared__ int32_t hval[kHashSlots][6];
+    if (*ws.error != 0) return;
+    for (int h = threadIdx.x; h < kHashSlots; h += blockDim.x) {
+        hkey[h] = kEmpty;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) { hval[h][a] = 0x7FFFFFFF; hval[h][3 + a] = (int32_t)0x80000000; }
+    }
+    __syncthreads();
+    const uint32_t I = N - 1;
+    const uint64_t pos = (uint64_t)blockIdx.x * kRefitLeaves + threadIdx.x;
+    if (pos < N) {
+        const uint32_t cur = I + (uint32_t)pos;
+        const kr_uvec4 face = F[nodes[cur].u.primitivesOffset];
+        const f3 v0 = ldv(V, face.x), v1 = ldv(V, face.y), v2 = ldv(V, face.z);
+        uint32_t p = parents[cur];
+        for (uint32_t step = 0; p != kInvalid && p < I && step < N; ++step) {
+            const Cand c = ws.cand[p];
+            if (!c.degen) {
+                int32_t k[6];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const f3 b = f3{c.b[3 * a], c.b[3 * a + 1], c.b[3 * a + 2]};
+                    const float d0 = dot3(v0, b), d1 = dot3(v1, b), d2 = dot3(v2, b);
+                    k[a] = f2ord(kmin(kmin(d0, d1), d2));
+                    k[3 + a] = f2ord(kmax(kmax(d0, d1), d2));
+                }
+                // find / claim the slot of p (linear probing)
+                uint32_t h = (p * 2654435761u) >> (32 - 11);
+                int slot = -1;
+                for (int probe = 0; probe < 64; ++probe) {
+                    uint32_t key = hkey[h];
+                    if (key == kEmpty) key = atomicCAS(&hkey[h], kEmpty, p);
+                    if (key == kEmpty || key == p) { slot = (int)h; break; }
+                    h = (h + 1) & (kHashSlots - 1);
+                }
+                if (slot >= 0) {
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        atomicMin(&hval[slot][a], k[a]);
+                        atomicMax(&hval[slot][3 + a], k[3 + a]);
+                    }
+                } else {
+                    fold_global(&ws.range[(size_t)p * 6], k);
+                }
             }
+            p = parents[p];
         }
-        p = parents[p];
+    }
+    __syncthreads();
+    for (int h = threadIdx.x; h < kHashSlots; h += blockDim.x) {
+        const uint32_t node = hkey[h];
+        if (node == kEmpty) continue;
+        int32_t k[6];
+#pragma unroll
+        for (int a = 0; a < 6; ++a) k[a] = hval[h][a];
+        fold_global(&ws.range[(size_t)node * 6], k);
     }
 }
 
@@ -566,11 +630,12 @@ __global__ void copy_ids(TreeWs ws, uint32_t* ids, uint32_t total) {
 size_t tree_carve(TreeWs* ws, void* base, size_t cap, uint32_t N) {
     Carver c{(char*)base, 0, cap};
     const size_t I = N ? N - 1 : 0, total = N ? 2 * (size_t)N - 1 : 0;
-    ws->visit = c.take<uint32_t>(I + 1);
-    ws->error = c.take<uint32_t>(4);
-    ws->ndef = ws->error ? ws->error + 1 : nullptr;
-    ws->vmax = ws->error ? ws->error + 2 : nullptr;
-    ws->deferred = c.take<uint32_t>(I + 1);
+    ws->visit = c.take<uint64_t>(I + 1);
+    ws->error = c.take<uint32_t>(8);
+    ws->vmax = ws->error ? ws->error + 1 : nullptr;
+    ws->nlist = ws->error ? ws->error + 2 : nullptr;
+    ws->list = c.take<uint32_t>((size_t)kMaxPhases * (I + 1));
+    ws->span = c.take<uint2>(total);
     ws->kdop = c.take<float4>(total * 7);
     ws->cand = c.take<Cand>(I + 1);
     ws->range = c.take<int32_t>((I + 1) * 6);
@@ -587,13 +652,19 @@ kr_error tree_launch(const kr_bvh_node* nodes, const uint32_t* parents, const kr
     const uint32_t I = N - 1;
     auto blocks = [](uint32_t n, uint32_t b) { return dim3((n + b - 1) / b); };
     hipLaunchKernelGGL(tree_clear, dim3(1024), dim3(256), 0, stream, ws.visit, I + 1, ws);
-    const uint32_t chunk = kClimbChunk;
-    hipLaunchKernelGGL(climb_local, blocks(N, chunk), dim3(1024), 0, stream, nodes, parents, V, F, obbs, mats, ws,
-                       N, chunk);
+    hipLaunchKernelGGL(leaf_boxes, blocks(N, 256), dim3(256), 0, stream, nodes, V, F, obbs, mats, ws, N);
+    // climb phases: 1024-leaf workgroups, then 32768-leaf workgroups, then one
+    uint32_t phase = 0;
+    for (uint32_t chunk : {kChunk0, kChunk1, 0u}) {
+        const bool last = (chunk == 0) || chunk >= N;
+        const uint32_t c = last ? N : chunk;
+        hipLaunchKernelGGL(climb_phase, blocks(N, c), dim3(1024), 0, stream, nodes, parents, V, F, ws, N, phase, c);
+        ++phase;
+        if (last) break;
+    }
     if (I > 0) {
-        hipLaunchKernelGGL(climb_top, dim3(1), dim3(1024), 0, stream, nodes, parents, ws, N);
         hipLaunchKernelGGL(candidates, blocks(I, 256), dim3(256), 0, stream, V, ws, I);
-        hipLaunchKernelGGL(refit, blocks(N, 256), dim3(256), 0, stream, nodes, parents, V, F, ws, N);
+        hipLaunchKernelGGL(refit, blocks(N, kRefitLeaves), dim3(kRefitLeaves), 0, stream, nodes, parents, V, F, ws, N);
         hipLaunchKernelGGL(finalize, blocks(I, 256), dim3(256), 0, stream, ws, obbs, mats, I);
     }
     if (ids) hipLaunchKernelGGL(copy_ids, blocks(2 * N - 1, 256), dim3(256), 0, stream, ws, ids, 2 * N - 1);

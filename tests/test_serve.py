@@ -1,0 +1,80 @@
+"""Serving loop: admission by HBM budget (CPU) and alone-vs-mix equality (GPU)."""
+import numpy as np
+import pytest
+
+from obvh_amd import serve
+
+
+class _Shape:
+    def __init__(self, *shape):
+        self.shape = shape
+
+
+def test_admission_respects_budget_and_order():
+    srv = serve.Server(hbm_budget=serve.fit_bytes(1000) * 3 + 10)
+    reqs = [serve.Request(i, "fit", 0.0, points=_Shape(3, 1000)) for i in range(5)]
+    for r in reqs:
+        srv.submit(r)
+    fits, trees = srv._admit()
+    assert [r.rid for r in fits] == [0, 1, 2] and trees == []
+    assert srv.in_use == 3 * serve.fit_bytes(1000)
+    assert [r.rid for r in srv.waiting] == [3, 4]
+    srv.in_use -= reqs[0].nbytes  # one result came back
+    fits, _ = srv._admit()
+    assert [r.rid for r in fits] == [3]
+
+
+def test_request_larger_than_budget_is_rejected():
+    srv = serve.Server(hbm_budget=1000)
+    with pytest.raises(ValueError):
+        srv.submit(serve.Request(0, "fit", 0.0, points=_Shape(3, 10 ** 6)))
+
+
+def test_batch_point_cap_splits_batches():
+    srv = serve.Server(hbm_budget=1e12, max_batch_points=2500)
+    for i in range(4):
+        srv.submit(serve.Request(i, "fit", 0.0, points=_Shape(3, 1000)))
+    first, _ = srv._admit()
+    second, _ = srv._admit()
+    assert [r.rid for r in first] == [0, 1] and [r.rid for r in second] == [2, 3]
+
+
+def test_poisson_schedule_rate_and_mix():
+    s = serve.poisson_schedule(2000.0, 5.0, seed=1)
+    assert abs(len(s) - 10000) < 400
+    assert all(a[0] < b[0] for a, b in zip(s, s[1:]))
+    share = sum(1 for x in s if x[1] == "fit") / len(s)
+    assert abs(share - 0.7) < 0.02
+
+
+@pytest.mark.gpu
+def test_results_alone_equal_results_in_the_mix(oracle_lib):
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    print("[serve] alone vs mix", flush=True)
+    fit_pool, tree_pool = serve.request_pool(seed=3, fits=12, trees=3, fit_range=(1000, 60000),
+                                             tree_range=(2000, 40000))
+    sched = serve.poisson_schedule(5000.0, 0.03, seed=4)  # ~150 requests, overlapping
+    mixed = {r.rid: r for r in serve.run_open_loop(serve.Server(), sched, fit_pool, tree_pool)}
+    assert len(mixed) == len(sched)
+    alone_srv = serve.Server()
+    for rid, (at, kind, pick) in enumerate(sched):
+        if kind == "fit":
+            r = serve.Request(rid, "fit", 0.0, points=fit_pool[pick % len(fit_pool)])
+        else:
+            V, F = tree_pool[pick % len(tree_pool)]
+            r = serve.Request(rid, "tree", 0.0, vertices=V, faces=F)
+        alone_srv.submit(r)
+        while not alone_srv.idle():
+            alone_srv.step(0.0)
+    alone = {r.rid: r for r in alone_srv.finished}
+    for rid, r in mixed.items():
+        a = alone[rid]
+        if r.kind == "fit":
+            assert np.array_equal(r.result, a.result)
+            assert np.array_equal(r.result, oracle_lib.points_obb(r.points.numpy().T))
+        else:
+            assert np.array_equal(r.result[0], a.result[0]) and np.array_equal(r.result[1], a.result[1])
+    assert mixed and max(len(x) for x in [mixed]) > 0
