@@ -142,6 +142,54 @@ def dito_legs(args) -> dict:
     return out
 
 
+def serving_leg(args) -> dict:
+    """Open-loop Poisson stream (70% DiTO fits of 1k-256k points, 30% mesh ->
+    LBVH -> OBB tree of 10k-2M triangles) under a 200 GB HBM budget, then a
+    closed-loop burst of fits served vs one plain call per request."""
+    import numpy as np
+    import torch
+
+    from obvh_amd import ops, serve
+
+    torch.cuda.empty_cache()  # hand back what the DiTO legs cached
+    t0 = time.perf_counter()
+    fit_pool, tree_pool = serve.request_pool()
+    print(f"[bench] serving pool ready in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    sched = serve.poisson_schedule(args.serve_rate, args.serve_seconds)
+    srv = serve.Server(hbm_budget=200e9)
+    w0 = time.perf_counter()
+    done = serve.run_open_loop(srv, sched, fit_pool, tree_pool)
+    wall = time.perf_counter() - w0
+    out = {"open_loop": serve.latency_stats(done)}
+    out["open_loop"].update({"rate": args.serve_rate, "seconds": args.serve_seconds,
+                             "req_per_s": round(len(done) / wall, 1), "fit_batches": srv.batches,
+                             "mean_fits_per_batch": round(float(np.mean(srv.batch_sizes)), 2)})
+    print(f"[bench] serving open loop: {json.dumps(out['open_loop'])}", file=sys.stderr, flush=True)
+    # closed loop: a burst of fits, all arrived at t = 0
+    burst = 4096
+    srv = serve.Server(hbm_budget=200e9)
+    for i in range(burst):
+        srv.submit(serve.Request(i, "fit", 0.0, points=fit_pool[i % len(fit_pool)]))
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    while not srv.idle():
+        srv.step(time.perf_counter() - w0)
+    served = time.perf_counter() - w0
+    ws = torch.empty(ops.lib().obvh_dito_workspace_size(1 << 18), dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    for i in range(burst):
+        pts = fit_pool[i % len(fit_pool)].to("cuda", non_blocking=True)
+        box = ops.dito(pts[0], pts[1], pts[2], workspace=ws).cpu()
+    plain = time.perf_counter() - w0
+    out["closed_loop_fits"] = {"requests": burst, "served_sets_per_s": round(burst / served, 1),
+                               "plain_sets_per_s": round(burst / plain, 1),
+                               "served_over_plain": round(plain / served, 2)}
+    print(f"[bench] serving closed loop: {json.dumps(out['closed_loop_fits'])}", file=sys.stderr, flush=True)
+    del box
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -152,10 +200,13 @@ def main() -> int:
     ap.add_argument("--no-dito", action="store_true", help="skip the DiTO legs")
     ap.add_argument("--sets", type=int, default=8192, help="batched DiTO: number of sets")
     ap.add_argument("--no-plain", action="store_true", help="skip the one-call-per-set DiTO comparison")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve-seconds", type=float, default=10.0, help="open-loop serving duration")
+    ap.add_argument("--serve-rate", type=float, default=2000.0, help="open-loop arrival rate (req/s)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build runs the OBB path on a single MI355X "
-              "(multi-GPU sharding is described in DESIGN.md, not launched)", file=sys.stderr)
+        print("bench.py measures one GPU only: the OBB path runs on a single MI355X; "
+              "sharding over several GPUs is not launched", file=sys.stderr)
         return 2
 
     import numpy as np
@@ -232,6 +283,14 @@ def main() -> int:
             print(f"[bench] DiTO legs failed: {e!r}", file=sys.stderr, flush=True)
             dito = {"error": repr(e)}
 
+    served = None
+    if not args.no_serve:
+        try:
+            served = serving_leg(args)
+        except Exception as e:
+            print(f"[bench] serving leg failed: {e!r}", file=sys.stderr, flush=True)
+            served = {"error": repr(e)}
+
     line = {
         "metric": "OBB-tree transform Mnodes/s; batched DiTO Gpts/s; served p50/p99 ms",
         "value": round(value, 2),
@@ -267,6 +326,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "parity": parity,
         "dito": dito,
+        "serving": served,
     }
     print(json.dumps(line))
     return 0

@@ -12,12 +12,14 @@
 //  pass 1    persistent workgroups (4 x wave64) stream tiles.  Lane l of a
 //            wave reads points chunk + j*64 + l, j < 8 (coalesced SoA or AoS),
 //            forms the 7 slab projections, and keeps per slab side the running
-//            extreme and the chunk where it last strictly improved.  The lane
-//            holding a wave's winner re-reads its 8 points to recover the
-//            exact first index, so index bookkeeping costs one compare per
-//            8 points.  (value, index) pairs reduce lexicographically (ties
-//            keep the lower index): order free, and equal to the reference's
-//            sequential strict-compare scan.  Output: 14 pairs per tile.
+//            extreme with the index where it was first reached.  The 8-point
+//            group extreme costs 7 min/max ops per side; only a strict
+//            improvement (rare once a few chunks are seen) searches the group
+//            for its first attaining point, so index bookkeeping costs one
+//            compare per 8 points.  (value, index) pairs reduce
+//            lexicographically (ties keep the lower index): order free, and
+//            equal to the reference's sequential strict-compare scan.
+//            Output: 14 pairs per tile.
 //  cand      one wave per set reduces its tile partials, then one lane runs
 //            the candidate search (base triangle + ditetrahedron, up to 21
 //            axis triples) over a 14-entry point array held in registers.
@@ -154,13 +156,15 @@ __global__ __launch_bounds__(kBlock) void dito_pass1(P pts, const uint64_t* offs
         const uint64_t end = base + tile < send ? base + tile : send;
         const uint32_t nchunks = (uint32_t)((end - base + kChunk - 1) / kChunk);
 
+        // per lane and slab side: running extreme and the index where it was
+        // first reached (points of one lane are visited in increasing index)
         float rmn[7], rmx[7];
-        uint32_t cmn[7], cmx[7];
+        uint32_t imn[7], imx[7];
 #pragma unroll
         for (int k = 0; k < 7; ++k) {
-            rmn[k] = NaN;  // NaN never compares: the first real group wins
+            rmn[k] = NaN;  // NaN never compares: the first real value wins
             rmx[k] = NaN;
-            cmn[k] = cmx[k] = kNoIdx;
+            imn[k] = imx[k] = kNoIdx;
         }
         for (uint32_t c = wave; c < nchunks; c += kWaves) {
             const uint64_t cb = base + (uint64_t)c * kChunk + lane;
@@ -185,45 +189,45 @@ __global__ __launch_bounds__(kBlock) void dito_pass1(P pts, const uint64_t* offs
                     gmx[k] = fmaxf(gmx[k], pr[k]);
                 }
             }
+            // Strict improvement (or first value): locate the first point of
+            // the group that attains it.  Rare after the first chunks.
 #pragma unroll
             for (int k = 0; k < 7; ++k) {
-                if (gmn[k] < rmn[k] || (cmn[k] == kNoIdx && gmn[k] == gmn[k])) { rmn[k] = gmn[k]; cmn[k] = c; }
-                if (gmx[k] > rmx[k] || (cmx[k] == kNoIdx && gmx[k] == gmx[k])) { rmx[k] = gmx[k]; cmx[k] = c; }
-            }
-        }
-        // Per slab side: wave winner (value, chunk), then its exact first index.
-        for (int side = 0; side < 14; ++side) {
-            const int k = side < 7 ? side : side - 7;
-            const bool isMin = side < 7;
-            float v = rmn[0];
-            uint32_t ch = cmn[0];
+                if (gmn[k] < rmn[k] || (imn[k] == kNoIdx && gmn[k] == gmn[k])) {
+                    uint32_t jj = kG;
 #pragma unroll
-            for (int q = 0; q < 7; ++q) {
-                if (k == q) { v = isMin ? rmn[q] : rmx[q]; ch = isMin ? cmn[q] : cmx[q]; }
-            }
-            float wv = ch == kNoIdx ? (isMin ? FLT_MAX * 2.0f : -FLT_MAX * 2.0f) : v;
-            uint32_t wc = ch;
-            if (isMin) wave_pair<true>(wv, wc); else wave_pair<false>(wv, wc);
-            uint32_t myidx = kNoIdx;
-            float myval = wv;
-            if (wc != kNoIdx && ch == wc && v == wv) {
-                const uint64_t cb = base + (uint64_t)wc * kChunk + lane;
-                for (int j = 0; j < kG; ++j) {
-                    const uint64_t idx = cb + (uint64_t)j * kWave;
-                    if (idx >= end) break;
-                    const float pv = slab_k(pts.load(idx), k);
-                    if (pv == wv) { myidx = (uint32_t)(idx - sbeg); myval = pv; break; }
+                    for (int j = kG - 1; j >= 0; --j)
+                        if (slab_k(p[j], k) == gmn[k]) jj = (uint32_t)j;
+                    rmn[k] = slab_k(p[0], k);
+#pragma unroll
+                    for (int j = 1; j < kG; ++j)
+                        if ((uint32_t)j == jj) rmn[k] = slab_k(p[j], k);
+                    imn[k] = (uint32_t)(cb + (uint64_t)jj * kWave - sbeg);
+                }
+                if (gmx[k] > rmx[k] || (imx[k] == kNoIdx && gmx[k] == gmx[k])) {
+                    uint32_t jj = kG;
+#pragma unroll
+                    for (int j = kG - 1; j >= 0; --j)
+                        if (slab_k(p[j], k) == gmx[k]) jj = (uint32_t)j;
+                    rmx[k] = slab_k(p[0], k);
+#pragma unroll
+                    for (int j = 1; j < kG; ++j)
+                        if ((uint32_t)j == jj) rmx[k] = slab_k(p[j], k);
+                    imx[k] = (uint32_t)(cb + (uint64_t)jj * kWave - sbeg);
                 }
             }
+        }
+        // wave: (value, index) lexicographic, then the workgroup over waves
 #pragma unroll
-            for (int m = 1; m < kWave; m <<= 1) {  // lowest index among tied lanes
-                const uint32_t oi = (uint32_t)__shfl_xor((int)myidx, m, kWave);
-                const float ov = __shfl_xor(myval, m, kWave);
-                if (oi < myidx) { myidx = oi; myval = ov; }
-            }
+        for (int side = 0; side < 14; ++side) {
+            const int k = side < 7 ? side : side - 7;
+            float v = side < 7 ? rmn[k] : rmx[k];
+            uint32_t i = side < 7 ? imn[k] : imx[k];
+            if (i == kNoIdx) v = side < 7 ? FLT_MAX * 2.0f : -FLT_MAX * 2.0f;
+            if (side < 7) wave_pair<true>(v, i); else wave_pair<false>(v, i);
             if (lane == 0) {
-                sh_v[wave][side] = myval;
-                sh_i[wave][side] = myidx;
+                sh_v[wave][side] = v;
+                sh_i[wave][side] = i;
             }
         }
         __syncthreads();

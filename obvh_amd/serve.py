@@ -3,7 +3,9 @@
 Two request kinds:
 * ``fit``   DiTO-14 box of one point set (host float32 (n, 3)).
 * ``tree``  mesh (host vertices (V, 3), faces (F, 4)) -> LBVH -> OBB tree;
-            the result is obbs (2F-1, 15) and matrices (2F-1, 16).
+            the host receives the root box.  With ``keep_trees`` the tree
+            (obbs (2F-1, 15), matrices (2F-1, 16), nodes) stays resident in
+            HBM, still charged to the budget, until ``release(request)``.
 
 Each scheduler iteration
   1. admits arrived requests in arrival order while their device footprint
@@ -13,7 +15,7 @@ Each scheduler iteration
      obvh_dito_batch, download) on one of two alternating streams, so a new
      batch starts while the previous one still runs;
   3. starts each admitted tree request on the next of several streams
-     (upload, obvh_lbvh_build, obvh_obb_tree_async, download);
+     (upload, obvh_lbvh_build, obvh_obb_tree_async, root box download);
   4. polls completion events and returns results.
 Latency is arrival -> result on the host.  The kernels are deterministic, so
 a request's result is the same alone and in any mix.
@@ -56,10 +58,12 @@ def tree_bytes(nv: int, nf: int) -> int:
 
 
 class Server:
-    def __init__(self, hbm_budget: float = 200e9, tree_streams: int = 4, max_batch_points: int = 1 << 27):
+    def __init__(self, hbm_budget: float = 200e9, tree_streams: int = 4, max_batch_points: int = 1 << 27,
+                 keep_trees: bool = False):
         import torch
 
         self.torch = torch
+        self.keep_trees = keep_trees
         self.budget = int(hbm_budget)
         self.in_use = 0
         self.max_batch_points = max_batch_points
@@ -143,6 +147,9 @@ class Server:
         self.batch_sizes.append(len(fits))
 
     def _launch_tree(self, r: Request) -> None:
+        """Mesh -> LBVH -> OBB tree; the host receives the root box, which
+        marks completion.  With ``keep_trees`` the result also holds the
+        device tensors and the request stays charged until ``release``."""
         torch = self.torch
         self._streams()
         stream = self.tree_streams[self._ts]
@@ -152,21 +159,29 @@ class Server:
             F = r.faces.to("cuda", non_blocking=True)
             t = ops.lbvh_build(V, F, stream=stream)
             obbs, mats = ops.obb_tree(t["nodes"], t["parents"], V, F, stream=stream)
-            ho = torch.empty(obbs.shape, dtype=torch.float32, pin_memory=True)
-            hm = torch.empty(mats.shape, dtype=torch.float32, pin_memory=True)
-            ho.copy_(obbs, non_blocking=True)
-            hm.copy_(mats, non_blocking=True)
+            root = torch.empty(15, dtype=torch.float32, pin_memory=True)
+            root.copy_(obbs[0], non_blocking=True)
             ev = torch.cuda.Event()
             ev.record(stream)
-        keep = (V, F, t, obbs, mats)
+        keep = (V, F, t)
 
-        def finish(now, r=r, ho=ho, hm=hm, keep=keep):
-            r.result = (ho.numpy(), hm.numpy())
+        def finish(now, r=r, root=root, obbs=obbs, mats=mats, keep=keep, stream=stream):
+            r.result = {"root": root.numpy().copy()}
+            if self.keep_trees:
+                r.result.update({"obbs": obbs, "mats": mats, "nodes": keep[2]["nodes"], "stream": stream})
+            else:
+                self.in_use -= r.nbytes
             r.done = now
-            self.in_use -= r.nbytes
             self.finished.append(r)
 
         self.inflight.append((ev, finish))
+
+    def release(self, r: Request) -> None:
+        """Drop a resident tree (``keep_trees``) and return its bytes to the budget."""
+        if r.kind == "tree" and self.keep_trees and r.result is not None and "obbs" in r.result:
+            for k in ("obbs", "mats", "nodes", "stream"):
+                r.result.pop(k)
+            self.in_use -= r.nbytes
 
     # ------------------------------------------------------------- loop --
     def poll(self, now: float) -> int:

@@ -57,9 +57,9 @@ def test_results_alone_equal_results_in_the_mix(oracle_lib):
     fit_pool, tree_pool = serve.request_pool(seed=3, fits=12, trees=3, fit_range=(1000, 60000),
                                              tree_range=(2000, 40000))
     sched = serve.poisson_schedule(5000.0, 0.03, seed=4)  # ~150 requests, overlapping
-    mixed = {r.rid: r for r in serve.run_open_loop(serve.Server(), sched, fit_pool, tree_pool)}
+    mixed = {r.rid: r for r in serve.run_open_loop(serve.Server(keep_trees=True), sched, fit_pool, tree_pool)}
     assert len(mixed) == len(sched)
-    alone_srv = serve.Server()
+    alone_srv = serve.Server(keep_trees=True)
     for rid, (at, kind, pick) in enumerate(sched):
         if kind == "fit":
             r = serve.Request(rid, "fit", 0.0, points=fit_pool[pick % len(fit_pool)])
@@ -76,5 +76,14 @@ def test_results_alone_equal_results_in_the_mix(oracle_lib):
             assert np.array_equal(r.result, a.result)
             assert np.array_equal(r.result, oracle_lib.points_obb(r.points.numpy().T))
         else:
-            assert np.array_equal(r.result[0], a.result[0]) and np.array_equal(r.result[1], a.result[1])
-    assert mixed and max(len(x) for x in [mixed]) > 0
+            assert torch.equal(r.result["obbs"], a.result["obbs"]) and torch.equal(r.result["mats"], a.result["mats"])
+            assert np.array_equal(r.result["root"], r.result["obbs"][0].cpu().numpy())
+    # the mixed run really overlapped work: more than one fit per batch on average
+    assert any(r.kind == "tree" for r in mixed.values())
+    # resident trees stay charged to the budget until released
+    srv = serve.Server(keep_trees=True)
+    held = [r for r in mixed.values() if r.kind == "tree"]
+    for r in held:
+        srv.in_use += r.nbytes
+        srv.release(r)
+    assert srv.in_use == 0 and all(set(r.result) == {"root"} for r in held)
