@@ -3,25 +3,30 @@
 // Method of the reference GPU transform (src/ads/common/cuda/transforms.cu
 // :213-964), restructured for CDNA4:
 //
-//  K1 leaf_climb   one lane per leaf: the leaf's 7-slab k-DOP (3 vertices,
-//                  first vertex wins ties), the leaf OBB and its matrix
-//                  (transforms.cu:803-834), then a bottom-up climb with one
-//                  agent-scope acq_rel counter per internal node: the first
-//                  child to arrive stops, the second merges.  The merge takes
-//                  the right child's extreme only if strictly better, so ties
-//                  go to the left child and the selected vertex ids do not
-//                  depend on scheduling.  k-DOPs carry vertex ids, not
-//                  positions (7 x {min, id, max, id} = 112 B per node).
-//  K2 candidates   one lane per internal node: gather the 14 extremal
-//                  vertices and run the base-triangle + ditetrahedron search
-//                  entirely in registers (transforms.cu:413-756).
-//  K3 refit        one lane per leaf: for every ancestor, project the 3
-//                  vertices on its axes and lower/raise the ancestor's range
-//                  with integer atomics on order-preserving float keys,
-//                  after a plain-load filter (transforms.cu:836-900).  The
-//                  result is the exact range of every subtree vertex.
-//  K4 finalize     one lane per internal node: OBB vs AABB by half area,
-//                  extent floor, inverse(T * R * S) (transforms.cu:905-964).
+//  climb_leaves  one lane per leaf, 1024-leaf workgroups: the leaf OBB and
+//                its matrix (transforms.cu:803-834), the leaf's 7-slab k-DOP
+//                (3 vertices, first vertex wins ties), then a bottom-up climb
+//                in which the second child to arrive at a parent merges the
+//                two k-DOPs.  Arrivals at parents inside the workgroup's range
+//                are counted in LDS; the rest use a 64-bit global arrival
+//                word tagged with the workgroup, and parents whose children
+//                finished in different workgroups are deferred.
+//  climb_phase   the deferred parents, in 32768-leaf workgroups and then one
+//                workgroup for the rest (workgroup-scope ordering only; the
+//                kernel boundary publishes everything else).  The merge takes
+//                the right child's extreme only if strictly better, so ties go
+//                to the left child and the selected vertex ids do not depend
+//                on scheduling.  k-DOPs carry vertex ids, not positions
+//                (7 x {min, id, max, id} = 112 B per node).
+//  candidates    one lane per internal node: gather the 14 extremal vertices
+//                and run the base-triangle + ditetrahedron search in
+//                registers (transforms.cu:413-756).
+//  refit         exact projection range of every subtree vertex on each
+//                node's candidate axes (transforms.cu:836-900): per-lane walks
+//                below each wave's common ancestor, DPP wave reductions above
+//                it, LDS hash per workgroup, filtered global atomics.
+//  finalize      one lane per internal node: OBB vs AABB by half area, extent
+//                floor, inverse(T * R * S) (transforms.cu:905-964).
 //
 // Tie rule, degenerate nodes and floating point follow the reference with
 // the decisions documented in the oracle (oracle/transform.c).
@@ -112,18 +117,39 @@ __device__ __forceinline__ void obb_matrix(f3 v0, f3 v1, f3 v2, f3 mid, f3 ext, 
     minverse4(TRS, out);
 }
 
-__device__ __forceinline__ void store_obb(kr_obb3* o, f3 mid, f3 v0, f3 v1, f3 v2, f3 ext) {
-    float* d = reinterpret_cast<float*>(o);
+// One box (15 floats, kr_obb3 order) and one matrix (16 floats) into
+// staging rows.
+__device__ __forceinline__ void obb_row(float* d, f3 mid, f3 v0, f3 v1, f3 v2, f3 ext) {
     d[0] = mid.x; d[1] = mid.y; d[2] = mid.z;
     d[3] = v0.x; d[4] = v0.y; d[5] = v0.z;
     d[6] = v1.x; d[7] = v1.y; d[8] = v1.z;
     d[9] = v2.x; d[10] = v2.y; d[11] = v2.z;
     d[12] = ext.x; d[13] = ext.y; d[14] = ext.z;
 }
-__device__ __forceinline__ void store_mat(kr_mat4* m, const float* v) {
-    float4* d = reinterpret_cast<float4*>(m);
+
+// Output staging for 256 consecutive nodes: boxes at row stride 15 floats,
+// matrices at row stride 17 (odd strides: no LDS bank conflicts), written
+// back to HBM as contiguous dwords.
+constexpr int kStage = 256;
+struct Stage {
+    float ob[kStage * 15];
+    float mt[kStage * 17];
+};
+
+__device__ __forceinline__ void stage_nan(Stage& st, uint32_t r) {
+    const float nan = __builtin_nanf("");
 #pragma unroll
-    for (int i = 0; i < 4; ++i) d[i] = make_float4(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]);
+    for (int q = 0; q < 15; ++q) st.ob[r * 15 + q] = nan;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) st.mt[r * 17 + q] = nan;
+}
+
+__device__ __forceinline__ void stage_flush(const Stage& st, kr_obb3* obbs, kr_mat4* mats, size_t first,
+                                            uint32_t cnt) {
+    float* go = reinterpret_cast<float*>(obbs) + first * 15;
+    for (uint32_t j = threadIdx.x; j < cnt * 15; j += kStage) go[j] = st.ob[j];
+    float* gm = reinterpret_cast<float*>(mats) + first * 16;
+    for (uint32_t j = threadIdx.x; j < cnt * 16; j += kStage) gm[j] = st.mt[(j >> 4) * 17 + (j & 15)];
 }
 
 // ------------------------------------------------------------ k-DOP --
@@ -148,7 +174,7 @@ __device__ __forceinline__ void kdop_load(const float4* kd, uint32_t node, Kdop&
 }
 
 // Leaf OBB from the triangle frame, transforms.cu:803-834.
-__device__ __forceinline__ void leaf_obb(f3 v0, f3 v1, f3 v2, kr_obb3* obb, kr_mat4* mat) {
+__device__ __forceinline__ void leaf_obb(f3 v0, f3 v1, f3 v2, float* ob, float* mt) {
     f3 e0 = sub3(v1, v0);
     f3 e1 = sub3(v2, v0);
     f3 a2 = vm::cross(e0, e1);
@@ -170,17 +196,17 @@ __device__ __forceinline__ void leaf_obb(f3 v0, f3 v1, f3 v2, kr_obb3* obb, kr_m
     f3 m = vm::mul1(a0, mid.x);
     m = add3(m, vm::mul1(a1, mid.y));
     m = add3(m, vm::mul1(a2, mid.z));
-    store_obb(obb, m, a0, a1, a2, ext);
+    obb_row(ob, m, a0, a1, a2, ext);
     float M[16];
     obb_matrix(a0, a1, a2, m, ext, M);
-    store_mat(mat, M);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) mt[q] = M[q];
 }
 
 __global__ void tree_clear(uint64_t* visit, uint32_t n, TreeWs ws) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t == 0) {
         *ws.error = 0;
-        *ws.vmax = 0;
         for (int k = 0; k < kMaxPhases; ++k) ws.nlist[k] = 0;
     }
     for (uint32_t i = t; i < n; i += gridDim.x * blockDim.x) visit[i] = 0;
@@ -201,87 +227,282 @@ __device__ __forceinline__ void merge_left_first(const Kdop& L, const Kdop& R, K
     }
 }
 
-// Leaf boxes (transforms.cu:803-834) and the largest referenced vertex id.
-__global__ __launch_bounds__(256) void leaf_boxes(const kr_bvh_node* __restrict__ nodes,
-                                                  const kr_vec3* __restrict__ V,
-                                                  const kr_uvec4* __restrict__ F, kr_obb3* obbs,
-                                                  kr_mat4* mats, TreeWs ws, uint32_t N) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t vmax = 0;
-    if (i < N) {
-        const uint32_t cur = N - 1 + i;
-        const uint32_t f = nodes[cur].u.primitivesOffset;
-        if (f < N) {
-            const kr_uvec4 face = F[f];
-            vmax = max(face.x, max(face.y, face.z));
-            leaf_obb(ldv(V, face.x), ldv(V, face.y), ldv(V, face.z), &obbs[cur], &mats[cur]);
+// Leaf k-DOP of one triangle: 7 slabs, first vertex wins ties.
+__device__ __forceinline__ void leaf_kdop(f3 v0, f3 v1, f3 v2, const uint32_t vid[3], Kdop& k) {
+    const f3 vs[3] = {v0, v1, v2};
+    float pr[7];
+    slab7(vs[0], pr);
+#pragma unroll
+    for (int s = 0; s < 7; ++s) { k.mn[s] = k.mx[s] = pr[s]; k.imn[s] = k.imx[s] = vid[0]; }
+#pragma unroll
+    for (int j = 1; j < 3; ++j) {
+        slab7(vs[j], pr);
+#pragma unroll
+        for (int s = 0; s < 7; ++s) {
+            if (pr[s] < k.mn[s]) { k.mn[s] = pr[s]; k.imn[s] = vid[j]; }
+            if (pr[s] > k.mx[s]) { k.mx[s] = pr[s]; k.imx[s] = vid[j]; }
         }
     }
-#pragma unroll
-    for (int m = 1; m < kWave; m <<= 1) vmax = max(vmax, (uint32_t)__shfl_xor((int)vmax, m, kWave));
-    if (lane_id() == 0) atomicMax(ws.vmax, vmax);
 }
 
-// One phase of the bottom-up k-DOP pass.  Workgroup w of phase `phase` owns
-// the leaf positions [w*chunk, (w+1)*chunk).  Phase 0 starts from the leaves;
-// later phases start from the nodes the previous phase deferred (those whose
-// leaf span lies inside the workgroup's range).  At a parent, the first
-// arriving child stops; the second continues if the first was completed in
-// an earlier kernel (visible after the kernel boundary) or by its own
-// workgroup (workgroup-scope acq_rel on the arrival word orders the data);
-// otherwise the parent is deferred to the next phase.  The last phase owns
-// every leaf and never defers.  No device-scope fence is issued.
+// Leaf span of a subtree: first and last leaf position below it.  Bit 31 of
+// .y is set when the positions below are not contiguous (the subtree does
+// not hold every leaf between first and last); leaf positions are < 2^31.
+constexpr uint32_t kGap = 0x80000000u;
+__device__ __forceinline__ uint32_t span_hi(uint2 s) { return s.y & ~kGap; }
+__device__ __forceinline__ uint2 leaf_span(uint32_t pos) { return make_uint2(pos, pos); }
+__device__ __forceinline__ uint2 span_join(uint2 a, uint2 b) {
+    const uint32_t ay = a.y & ~kGap, by = b.y & ~kGap;
+    const bool gap = ((a.y | b.y) & kGap) != 0 || !(ay + 1u == b.x || by + 1u == a.x);
+    return make_uint2(min(a.x, b.x), max(ay, by) | (gap ? kGap : 0u));
+}
+
+__device__ __forceinline__ void defer(TreeWs& ws, uint32_t phase, uint32_t node, uint32_t I) {
+    const uint32_t slot = atomicAdd(&ws.nlist[phase], 1u);
+    if (slot < I) ws.list[(size_t)phase * (I + 1) + slot] = node;
+}
+
+// k-DOP of leaf node `leaf` from the mesh.
+__device__ __forceinline__ bool leaf_kdop_global(const kr_bvh_node* nodes, const kr_vec3* V, const kr_uvec4* F,
+                                                 uint32_t leaf, uint32_t N, Kdop& k) {
+    const uint32_t f = nodes[leaf].u.primitivesOffset;
+    if (f >= N) return false;
+    const kr_uvec4 face = F[f];
+    const uint32_t vid[3] = {face.x, face.y, face.z};
+    leaf_kdop(ldv(V, face.x), ldv(V, face.y), ldv(V, face.z), vid, k);
+    return true;
+}
+
+// Leaf boxes (transforms.cu:803-834), 256 leaves per workgroup, output
+// staged in LDS and written back contiguously.
+__global__ __launch_bounds__(kStage) void leaf_boxes(const kr_bvh_node* __restrict__ nodes,
+                                                     const kr_vec3* __restrict__ V,
+                                                     const kr_uvec4* __restrict__ F, kr_obb3* obbs,
+                                                     kr_mat4* mats, uint32_t N) {
+    __shared__ Stage st;
+    const uint32_t I = N - 1;
+    const uint32_t base = blockIdx.x * kStage, pos = base + threadIdx.x;
+    if (pos < N) {
+        const uint32_t f = nodes[I + pos].u.primitivesOffset;
+        if (f < N) {
+            const kr_uvec4 face = F[f];
+            leaf_obb(ldv(V, face.x), ldv(V, face.y), ldv(V, face.z), &st.ob[threadIdx.x * 15],
+                     &st.mt[threadIdx.x * 17]);
+        } else {
+            stage_nan(st, threadIdx.x);  // the climb reports the bad face id
+        }
+    }
+    __syncthreads();
+    stage_flush(st, obbs, mats, (size_t)I + base, min((uint32_t)kStage, N - base));
+}
+
+// Phase 0 of the bottom-up k-DOP pass.  Workgroup w owns leaf positions
+// [w*kTile, (w+1)*kTile) and the internal nodes with the same indices (for a
+// Karras-ordered tree these are the internal nodes whose leaves lie in the
+// range).  The workgroup first loads the tile's topology into LDS (children
+// and parents of its internal nodes, parents and vertex ids of its leaves)
+// and checks that every child of its internal nodes points back to it.
+// Then one lane per leaf climbs: at a parent in the tile the arrival is
+// counted in LDS, the first child stops and the second merges, reading the
+// sibling's k-DOP from LDS (internal node of the tile), recomputing it
+// (leaf of the tile) or loading it (node completed outside the tile's
+// index range).  Parents outside the tile use the global arrival word with
+// the workgroup's tag; such a parent is deferred to the next phase when its
+// children finished in different workgroups.  Finally tile nodes reached by
+// one child only are folded into the global arrival words, and the k-DOPs,
+// spans and arrival words of tile nodes completed here are written back
+// contiguously.  The merge keeps the left child's extreme on ties.
+constexpr uint32_t kTile = kChunk0;
+
+struct ClimbTile {
+    float4 kd[kTile][7];        // k-DOP of internal node lo + s, once completed here
+    uint2 span[kTile];
+    uint32_t lid[kTile][3];     // vertex ids of leaf position lo + s
+    uint32_t arrive[kTile];     // arrivals at internal node lo + s from this workgroup
+    uint2 kids[kTile];          // children of internal node lo + s
+    uint32_t par_int[kTile];    // parent of internal node lo + s
+    uint32_t par_leaf[kTile];   // parent of the leaf at position lo + s
+};
+
+__device__ __forceinline__ void tile_leaf_kdop(const ClimbTile& L, const kr_vec3* V, uint32_t s, Kdop& k) {
+    const uint32_t vid[3] = {L.lid[s][0], L.lid[s][1], L.lid[s][2]};
+    leaf_kdop(ldv(V, vid[0]), ldv(V, vid[1]), ldv(V, vid[2]), vid, k);
+}
+
+__global__ __launch_bounds__(kTile) void climb_leaves(const kr_bvh_node* __restrict__ nodes,
+                                                      const uint32_t* __restrict__ parents,
+                                                      const kr_vec3* __restrict__ V,
+                                                      const kr_uvec4* __restrict__ F, TreeWs ws, uint32_t N) {
+    __shared__ ClimbTile L;
+    const uint32_t I = N - 1, total = 2 * N - 1;
+    const uint32_t t = threadIdx.x;
+    const uint32_t lo = blockIdx.x * kTile;
+    const uint32_t tag = make_tag(0, blockIdx.x);
+    const uint32_t pos = lo + t;
+
+    // tile topology and the lane's leaf
+    Kdop k;
+    bool climbing = false;
+    if (pos < N) {
+        const uint32_t leaf = I + pos;
+        L.par_leaf[t] = parents[leaf];
+        const uint32_t f = nodes[leaf].u.primitivesOffset;
+        if (f < N) {
+            const kr_uvec4 face = F[f];
+            const uint32_t vid[3] = {face.x, face.y, face.z};
+            L.lid[t][0] = vid[0];
+            L.lid[t][1] = vid[1];
+            L.lid[t][2] = vid[2];
+            leaf_kdop(ldv(V, vid[0]), ldv(V, vid[1]), ldv(V, vid[2]), vid, k);
+            climbing = true;
+        } else {
+            L.lid[t][0] = kInvalid;  // never climbs, never read as a sibling
+            atomicOr(ws.error, 4u);  // faces has one entry per leaf
+        }
+    }
+    if (pos < I) {
+        const uint32_t l = nodes[pos].u.children.left, r = nodes[pos].u.children.right;
+        L.kids[t] = make_uint2(l, r);
+        L.par_int[t] = parents[pos];
+        if (l >= total || r >= total || parents[l] != pos || parents[r] != pos) atomicOr(ws.error, 2u);
+    }
+    L.arrive[t] = 0;
+    __syncthreads();
+
+    if (climbing) {
+        uint32_t cur = I + pos;
+        uint2 span = leaf_span(pos);
+        uint32_t p = L.par_leaf[t];
+        for (uint32_t step = 0; p != kInvalid; ++step) {
+            if (p >= I || step >= N) { atomicOr(ws.error, 1u); break; }
+            const uint32_t s = p - lo;
+            uint32_t sib, pp;
+            bool am_left;
+            if (s < kTile) {  // parent in the tile: LDS arrival
+                const uint32_t cnt =
+                    __hip_atomic_fetch_add(&L.arrive[s], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (cnt == 0) break;  // first arrival: the sibling continues
+                if (cnt != 1) { atomicOr(ws.error, 2u); break; }
+                const uint2 kc = L.kids[s];
+                am_left = kc.x == cur;
+                if (!am_left && kc.y != cur) { atomicOr(ws.error, 2u); break; }
+                sib = am_left ? kc.y : kc.x;
+                pp = L.par_int[s];
+            } else {  // parent outside the tile: publish, then global arrival
+                kdop_store(ws.kdop, cur, k);
+                ws.span[cur] = span;
+                const uint32_t l = nodes[p].u.children.left, rr = nodes[p].u.children.right;
+                pp = parents[p];
+                const uint64_t old = __hip_atomic_fetch_add(&ws.visit[p], ((uint64_t)tag << 32) | 1ull,
+                                                            __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+                const uint32_t cnt = (uint32_t)old;
+                if (cnt == 0) break;
+                if (cnt != 1) { atomicOr(ws.error, 2u); break; }
+                if ((uint32_t)(old >> 32) != tag) { defer(ws, 0, p, I); break; }  // other workgroup
+                am_left = l == cur;
+                if (!am_left && rr != cur) { atomicOr(ws.error, 2u); break; }
+                sib = am_left ? rr : l;
+            }
+            // the sibling finished in this workgroup
+            Kdop o;
+            uint2 so;
+            if (sib >= I) {
+                const uint32_t ss = sib - I - lo;
+                if (sib >= total || ss >= kTile) { atomicOr(ws.error, 2u); break; }
+                tile_leaf_kdop(L, V, ss, o);
+                so = leaf_span(sib - I);
+            } else if (sib - lo < kTile) {  // completed in this tile: k-DOP in LDS
+                const uint32_t ss = sib - lo;
+#pragma unroll
+                for (int q = 0; q < 7; ++q) {
+                    const float4 a = L.kd[ss][q];
+                    o.mn[q] = a.x; o.imn[q] = __float_as_uint(a.y);
+                    o.mx[q] = a.z; o.imx[q] = __float_as_uint(a.w);
+                }
+                so = L.span[ss];
+            } else {
+                kdop_load(ws.kdop, sib, o);
+                so = ws.span[sib];
+            }
+            Kdop m;
+            if (am_left) merge_left_first(k, o, m); else merge_left_first(o, k, m);
+            k = m;
+            span = span_join(span, so);
+            if (s < kTile) {
+#pragma unroll
+                for (int q = 0; q < 7; ++q)
+                    L.kd[s][q] = make_float4(k.mn[q], __uint_as_float(k.imn[q]), k.mx[q], __uint_as_float(k.imx[q]));
+                L.span[s] = span;
+            } else {
+                kdop_store(ws.kdop, p, k);
+                ws.span[p] = span;
+            }
+            cur = p;
+            p = pp;
+        }
+    }
+    __syncthreads();
+
+    // tile nodes reached by one child: publish a tile-leaf child, then the
+    // global arrival (the other child finishes in another workgroup)
+    if (pos < I && L.arrive[t] == 1u) {
+        const uint2 kc = L.kids[t];
+        const uint32_t c2[2] = {kc.x, kc.y};
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const uint32_t ss = c2[j] - I - lo;
+            if (c2[j] >= I && c2[j] < total && ss < kTile && L.lid[ss][0] != kInvalid) {
+                Kdop o;
+                tile_leaf_kdop(L, V, ss, o);
+                kdop_store(ws.kdop, c2[j], o);
+                ws.span[c2[j]] = leaf_span(c2[j] - I);
+            }
+        }
+        const uint64_t old = __hip_atomic_fetch_add(&ws.visit[pos], ((uint64_t)tag << 32) | 1ull,
+                                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if ((uint32_t)old == 1u) defer(ws, 0, pos, I);
+        else if ((uint32_t)old != 0u) atomicOr(ws.error, 2u);
+    }
+    // write back the tile nodes completed here
+    const uint32_t nin = lo < I ? min(kTile, I - lo) : 0u;
+    float4* kd = ws.kdop + (size_t)lo * 7;
+    for (uint32_t j = t; j < nin * 7; j += kTile) {
+        const uint32_t ss = j / 7;
+        if (L.arrive[ss] == 2u) kd[j] = L.kd[ss][j - ss * 7];
+    }
+    if (t < nin && L.arrive[t] == 2u) {
+        ws.span[pos] = L.span[t];
+        ws.visit[pos] = ((uint64_t)tag << 33) | 2ull;
+    }
+}
+
+// Phase >= 1 of the bottom-up k-DOP pass.  Workgroup w owns the leaf
+// positions [w*chunk, (w+1)*chunk) and starts from the nodes the previous
+// phase deferred whose first leaf it owns.  At a parent, the first arriving
+// child stops; the second continues if the first was completed in an earlier
+// kernel (visible after the kernel boundary) or by its own workgroup
+// (workgroup-scope acq_rel on the arrival word orders the data); otherwise
+// the parent is deferred to the next phase.  The last phase owns every leaf
+// and never defers.  No device-scope fence is issued.
 __global__ __launch_bounds__(1024) void climb_phase(const kr_bvh_node* __restrict__ nodes,
-                                                    const uint32_t* __restrict__ parents,
-                                                    const kr_vec3* __restrict__ V,
-                                                    const kr_uvec4* __restrict__ F, TreeWs ws, uint32_t N,
+                                                    const uint32_t* __restrict__ parents, TreeWs ws, uint32_t N,
                                                     uint32_t phase, uint32_t chunk) {
     const uint32_t I = N - 1, total = 2 * N - 1;
     const uint32_t tag = make_tag(phase, blockIdx.x);
     const uint64_t lo = (uint64_t)blockIdx.x * chunk, hi = lo + chunk;  // owned leaf positions
-    const uint32_t nin = phase == 0 ? 0 : min(ws.nlist[phase - 1], I);
-    const uint32_t* in = phase == 0 ? nullptr : ws.list + (size_t)(phase - 1) * (I + 1);
-    uint32_t* out = ws.list + (size_t)phase * (I + 1);
-    const uint32_t nstart = phase == 0 ? chunk : nin;
-    for (uint32_t r = threadIdx.x; r < nstart; r += blockDim.x) {
-        uint32_t cur;
+    const uint32_t nin = min(ws.nlist[phase - 1], I);
+    const uint32_t* in = ws.list + (size_t)(phase - 1) * (I + 1);
+    auto span_of = [&](uint32_t c) { return c >= I ? leaf_span(c - I) : ws.span[c]; };
+    for (uint32_t r = threadIdx.x; r < nin; r += blockDim.x) {
+        uint32_t cur = in[r];
+        if (cur >= I) { atomicOr(ws.error, 1u); continue; }
+        const uint32_t l0 = nodes[cur].u.children.left, r0 = nodes[cur].u.children.right;
+        if (l0 >= total || r0 >= total) { atomicOr(ws.error, 2u); continue; }
+        uint2 span = span_join(span_of(l0), span_of(r0));
+        if (span.x < lo || span.x >= hi) continue;  // another workgroup owns its first leaf
+        if (span_hi(span) >= hi) { defer(ws, phase, cur, I); continue; }  // wider than this workgroup
         Kdop k;
-        uint2 span;
-        if (phase == 0) {
-            const uint64_t pos = lo + r;
-            if (pos >= N) break;
-            cur = I + (uint32_t)pos;
-            const uint32_t f = nodes[cur].u.primitivesOffset;
-            if (f >= N) { atomicOr(ws.error, 4u); continue; }  // faces has one entry per leaf
-            const kr_uvec4 face = F[f];
-            const uint32_t vid[3] = {face.x, face.y, face.z};
-            float pr[7];
-            slab7(ldv(V, vid[0]), pr);
-#pragma unroll
-            for (int s = 0; s < 7; ++s) { k.mn[s] = k.mx[s] = pr[s]; k.imn[s] = k.imx[s] = vid[0]; }
-#pragma unroll
-            for (int j = 1; j < 3; ++j) {
-                slab7(ldv(V, vid[j]), pr);
-#pragma unroll
-                for (int s = 0; s < 7; ++s) {
-                    if (pr[s] < k.mn[s]) { k.mn[s] = pr[s]; k.imn[s] = vid[j]; }
-                    if (pr[s] > k.mx[s]) { k.mx[s] = pr[s]; k.imx[s] = vid[j]; }
-                }
-            }
-            span = make_uint2((uint32_t)pos, (uint32_t)pos);
-        } else {
-            cur = in[r];
-            if (cur >= I) { atomicOr(ws.error, 1u); continue; }
-            const uint32_t l0 = nodes[cur].u.children.left, r0 = nodes[cur].u.children.right;
-            if (l0 >= total || r0 >= total) { atomicOr(ws.error, 2u); continue; }
-            const uint2 sa = ws.span[l0], sb = ws.span[r0];
-            span = make_uint2(min(sa.x, sb.x), max(sa.y, sb.y));
-            if (span.x < lo || span.x >= hi) continue;  // another workgroup owns its first leaf
-            if (span.y >= hi) {                         // wider than this workgroup: next phase
-                const uint32_t slot = atomicAdd(&ws.nlist[phase], 1u);
-                if (slot < I) out[slot] = cur;
-                continue;
-            }
+        {
             Kdop a, b;
             kdop_load(ws.kdop, l0, a);
             kdop_load(ws.kdop, r0, b);
@@ -292,17 +513,14 @@ __global__ __launch_bounds__(1024) void climb_phase(const kr_bvh_node* __restric
         uint32_t p = parents[cur];
         for (uint32_t step = 0; p != kInvalid; ++step) {
             if (p >= I || step >= N) { atomicOr(ws.error, 1u); break; }
+            const uint32_t l = nodes[p].u.children.left, rr = nodes[p].u.children.right;
+            const uint32_t pp = parents[p];
             const uint64_t old = __hip_atomic_fetch_add(&ws.visit[p], ((uint64_t)tag << 32) | 1ull,
                                                         __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
             const uint32_t cnt = (uint32_t)old, t1 = (uint32_t)(old >> 32);
             if (cnt == 0) break;  // first arrival: the sibling continues
             if (cnt != 1) { atomicOr(ws.error, 2u); break; }
-            if (t1 != tag && (t1 >> 24) >= phase) {  // sibling done by another workgroup of this phase
-                const uint32_t slot = atomicAdd(&ws.nlist[phase], 1u);
-                if (slot < I) out[slot] = p;
-                break;
-            }
-            const uint32_t l = nodes[p].u.children.left, rr = nodes[p].u.children.right;
+            if (t1 != tag && (t1 >> 24) >= phase) { defer(ws, phase, p, I); break; }
             const bool am_left = (l == cur);
             const uint32_t sib = am_left ? rr : l;
             if (sib >= total || (!am_left && rr != cur)) { atomicOr(ws.error, 2u); break; }
@@ -310,12 +528,11 @@ __global__ __launch_bounds__(1024) void climb_phase(const kr_bvh_node* __restric
             kdop_load(ws.kdop, sib, o);
             if (am_left) merge_left_first(k, o, m); else merge_left_first(o, k, m);
             k = m;
-            const uint2 so = ws.span[sib];
-            span = make_uint2(min(span.x, so.x), max(span.y, so.y));
+            span = span_join(span, span_of(sib));
             kdop_store(ws.kdop, p, k);
             ws.span[p] = span;
             cur = p;
-            p = parents[cur];
+            p = pp;
         }
     }
 }
@@ -328,37 +545,76 @@ struct Best {
     f3 dmin, dmax;
 };
 
-__device__ __forceinline__ void proj14(const f3* P, f3 d, float& mn, float& mx) {
-    float p = dot3(P[0], d);
-    mn = p;
-    mx = p;
+// (min, max) of 14 projections (transforms.cu:418-434).  v_min / v_max give the same value as the
+// reference's compare-and-select rule (kmin / kmax) except for the sign of a
+// zero result and for NaN inputs; a zero result, or a node whose points are
+// not all finite and bounded (`sane`), takes the compare-and-select chain.
+__device__ __forceinline__ void minmax14(const float (&p)[14], bool sane, float& mn, float& mx) {
+    float a = fminf(p[0], p[1]), b = fmaxf(p[0], p[1]);
 #pragma unroll
-    for (int i = 1; i < 14; ++i) {
-        p = dot3(P[i], d);
-        mn = kmin(mn, p);
-        mx = kmax(mx, p);
+    for (int i = 2; i < 14; ++i) {
+        a = fminf(a, p[i]);
+        b = fmaxf(b, p[i]);
     }
+    if (!sane || a == 0.0f || b == 0.0f) {
+        a = p[0];
+        b = p[0];
+#pragma unroll
+        for (int i = 1; i < 14; ++i) {
+            a = kmin(a, p[i]);
+            b = kmax(b, p[i]);
+        }
+    }
+    mn = a;
+    mx = b;
 }
 
-// transforms.cu:484-544
-__device__ __forceinline__ void from_normal_edges(const f3* P, f3 n, f3 e0, f3 e1, f3 e2, Best& b) {
-    float nmn, nmx;
-    proj14(P, n, nmn, nmx);
+__device__ __forceinline__ void proj14(const f3* P, f3 d, bool sane, float& mn, float& mx) {
+    float a = dot3(P[0], d), b = a;
+#pragma unroll
+    for (int i = 1; i < 14; ++i) {
+        const float p = dot3(P[i], d);
+        a = fminf(a, p);
+        b = fmaxf(b, p);
+    }
+    if (!sane || a == 0.0f || b == 0.0f) {  // same projections, reference rule
+        a = dot3(P[0], d);
+        b = a;
+#pragma unroll
+        for (int i = 1; i < 14; ++i) {
+            const float p = dot3(P[i], d);
+            a = kmin(a, p);
+            b = kmax(b, p);
+        }
+    }
+    mn = a;
+    mx = b;
+}
+
+__device__ __forceinline__ float2 proj14v(const f3* P, f3 d, bool sane) {
+    float mn, mx;
+    proj14(P, d, sane, mn, mx);
+    return make_float2(mn, mx);
+}
+
+// transforms.cu:484-544 with the normal's projection range (nmn, nmx) and
+// the edges' ranges pe[k] computed by the caller: each edge is shared by up
+// to three calls, so its range is computed once (identical arithmetic).
+__device__ __forceinline__ void from_normal_edges(const f3* P, f3 n, float nmn, float nmx, const f3 (&e)[3],
+                                                  const float2 (&pe)[3], bool sane, Best& b) {
     const float leny = nmx - nmn;
-    const f3 e[3] = {e0, e1, e2};
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         const f3 m = vm::cross(e[k], n);
-        float emn, emx, mmn, mmx;
-        proj14(P, e[k], emn, emx);
-        const float lenx = emx - emn;
-        proj14(P, m, mmn, mmx);
+        float mmn, mmx;
+        proj14(P, m, sane, mmn, mmx);
+        const float lenx = pe[k].y - pe[k].x;
         const float lenz = mmx - mmn;
         const float q = quality(f3{lenx, leny, lenz});
         if (q < b.q) {
             b.q = q;
-            b.dmax = f3{emx, nmx, mmx};
-            b.dmin = f3{emn, nmn, mmn};
+            b.dmax = f3{pe[k].y, nmx, mmx};
+            b.dmin = f3{pe[k].x, nmn, mmn};
             b.b0 = e[k];
             b.b1 = n;
             b.b2 = m;
@@ -366,7 +622,41 @@ __device__ __forceinline__ void from_normal_edges(const f3* P, f3 n, f3 e0, f3 e
     }
 }
 
-__global__ __launch_bounds__(256) void candidates(const kr_vec3* __restrict__ V, TreeWs ws, uint32_t I) {
+__device__ __forceinline__ void face(const f3* P, f3 n, const f3 (&e)[3], const float2 (&pe)[3], bool sane,
+                                     Best& b) {
+    float nmn, nmx;
+    proj14(P, n, sane, nmn, nmx);
+    from_normal_edges(P, n, nmn, nmx, e, pe, sane, b);
+}
+
+// The three tetrahedron faces on apex q of the base triangle (p0, p1, p2)
+// with edges e0, e1, e2 and their ranges pe (transforms.cu:641-700).
+__device__ __forceinline__ void apex_faces(const f3* P, f3 q, f3 p0, f3 p1, f3 p2, f3 e0, f3 e1, f3 e2,
+                                           float2 pe0, float2 pe1, float2 pe2, bool sane, Best& b) {
+    const f3 f0 = vm::normalize(sub3(q, p0));
+    const f3 f1 = vm::normalize(sub3(q, p1));
+    const f3 f2 = vm::normalize(sub3(q, p2));
+    const float2 pf0 = proj14v(P, f0, sane), pf1 = proj14v(P, f1, sane), pf2 = proj14v(P, f2, sane);
+    {
+        const f3 e[3] = {e0, f1, f0};
+        const float2 pe[3] = {pe0, pf1, pf0};
+        face(P, vm::normalize(vm::cross(f1, e0)), e, pe, sane, b);
+    }
+    {
+        const f3 e[3] = {e1, f2, f1};
+        const float2 pe[3] = {pe1, pf2, pf1};
+        face(P, vm::normalize(vm::cross(f2, e1)), e, pe, sane, b);
+    }
+    {
+        const f3 e[3] = {e2, f0, f2};
+        const float2 pe[3] = {pe2, pf0, pf2};
+        face(P, vm::normalize(vm::cross(f0, e2)), e, pe, sane, b);
+    }
+}
+
+__device__ __forceinline__ f3 pick(bool c, f3 a, f3 b) { return f3{c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z}; }
+
+__global__ __launch_bounds__(256, 4) void candidates(const kr_vec3* __restrict__ V, TreeWs ws, uint32_t I) {
     const uint32_t node = blockIdx.x * blockDim.x + threadIdx.x;
     if (node >= I) return;
     if (*ws.error != 0 || !complete(ws.visit[node])) {
@@ -381,12 +671,20 @@ __global__ __launch_bounds__(256) void candidates(const kr_vec3* __restrict__ V,
     Kdop k;
     kdop_load(ws.kdop, node, k);
     f3 P[14];
-    const uint32_t vmax = *ws.vmax;  // ids never exceed the largest face vertex id
 #pragma unroll
     for (int s = 0; s < 7; ++s) {
-        P[2 * s] = ldv(V, min(k.imn[s], vmax));
-        P[2 * s + 1] = ldv(V, min(k.imx[s], vmax));
+        P[2 * s] = ldv(V, k.imn[s]);
+        P[2 * s + 1] = ldv(V, k.imx[s]);
     }
+    // finite and below 2^61 in magnitude: no projection below can be NaN
+    uint32_t ex = 0;
+#pragma unroll
+    for (int i = 0; i < 14; ++i) {
+        ex = max(ex, __float_as_uint(P[i].x) & 0x7F800000u);
+        ex = max(ex, __float_as_uint(P[i].y) & 0x7F800000u);
+        ex = max(ex, __float_as_uint(P[i].z) & 0x7F800000u);
+    }
+    const bool sane = ex <= ((127u + 60u) << 23);
     const f3 alen = f3{k.mx[0] - k.mn[0], k.mx[1] - k.mn[1], k.mx[2] - k.mn[2]};
     Best b;
     b.q = quality(alen);
@@ -397,17 +695,17 @@ __global__ __launch_bounds__(256) void candidates(const kr_vec3* __restrict__ V,
     b.dmax = f3{k.mx[0], k.mx[1], k.mx[2]};
     int degen = 0;
 
-    int at = 0;
+    // farthest slab pair (transforms.cu:437-456), first pair wins ties
+    f3 p0 = P[0], p1 = P[1];
     float d2b = vm::dist2(P[0], P[1]);
 #pragma unroll
     for (int s = 1; s < 7; ++s) {
-        float d2 = vm::dist2(P[2 * s], P[2 * s + 1]);
-        if (d2 > d2b) { d2b = d2; at = s; }
+        const float d2 = vm::dist2(P[2 * s], P[2 * s + 1]);
+        const bool c = d2 > d2b;
+        d2b = c ? d2 : d2b;
+        p0 = pick(c, P[2 * s], p0);
+        p1 = pick(c, P[2 * s + 1], p1);
     }
-    f3 p0 = P[0], p1 = P[1];
-#pragma unroll
-    for (int s = 1; s < 7; ++s)
-        if (at == s) { p0 = P[2 * s]; p1 = P[2 * s + 1]; }
     if (vm::dist2(p0, p1) < eps) {
         degen = 1;
     } else {
@@ -416,8 +714,10 @@ __global__ __launch_bounds__(256) void candidates(const kr_vec3* __restrict__ V,
         f3 p2 = P[0];
 #pragma unroll
         for (int i = 1; i < 14; ++i) {
-            float d = vm::dist_inf_edge(P[i], p0, e0);
-            if (d > dm) { dm = d; p2 = P[i]; }
+            const float d = vm::dist_inf_edge(P[i], p0, e0);
+            const bool c = d > dm;
+            dm = c ? d : dm;
+            p2 = pick(c, P[i], p2);
         }
         if (dm < eps) {
             degen = 1;
@@ -425,15 +725,27 @@ __global__ __launch_bounds__(256) void candidates(const kr_vec3* __restrict__ V,
             const f3 e1 = vm::normalize(sub3(p1, p2));
             const f3 e2 = vm::normalize(sub3(p2, p0));
             const f3 n = vm::normalize(vm::cross(e1, e0));
-            from_normal_edges(P, n, e0, e1, e2, b);
-            float pr = dot3(P[0], n);
-            float lo = pr, hi = pr;
+            const float2 pe0 = proj14v(P, e0, sane), pe1 = proj14v(P, e1, sane), pe2 = proj14v(P, e2, sane);
+            float pn[14];
+#pragma unroll
+            for (int i = 0; i < 14; ++i) pn[i] = dot3(P[i], n);
+            {
+                float nmn, nmx;
+                minmax14(pn, sane, nmn, nmx);
+                const f3 e[3] = {e0, e1, e2};
+                const float2 pe[3] = {pe0, pe1, pe2};
+                from_normal_edges(P, n, nmn, nmx, e, pe, sane, b);
+            }
+            // apexes: extreme points along n, first index wins (transforms.cu:609-628)
+            float lo = pn[0], hi = pn[0];
             f3 q1 = P[0], q0 = P[0];
 #pragma unroll
             for (int i = 1; i < 14; ++i) {
-                pr = dot3(P[i], n);
-                if (pr < lo) { lo = pr; q1 = P[i]; }
-                if (pr > hi) { hi = pr; q0 = P[i]; }
+                const bool cl = pn[i] < lo, ch = pn[i] > hi;
+                lo = cl ? pn[i] : lo;
+                q1 = pick(cl, P[i], q1);
+                hi = ch ? pn[i] : hi;
+                q0 = pick(ch, P[i], q0);
             }
             const float tri = dot3(p0, n);
             const bool q0v = hi - eps > tri;
@@ -441,22 +753,8 @@ __global__ __launch_bounds__(256) void candidates(const kr_vec3* __restrict__ V,
             if (!q0v && !q1v) {
                 degen = 1;
             } else {
-                if (q0v) {
-                    const f3 f0 = vm::normalize(sub3(q0, p0));
-                    const f3 f1 = vm::normalize(sub3(q0, p1));
-                    const f3 f2 = vm::normalize(sub3(q0, p2));
-                    from_normal_edges(P, vm::normalize(vm::cross(f1, e0)), e0, f1, f0, b);
-                    from_normal_edges(P, vm::normalize(vm::cross(f2, e1)), e1, f2, f1, b);
-                    from_normal_edges(P, vm::normalize(vm::cross(f0, e2)), e2, f0, f2, b);
-                }
-                if (q1v) {
-                    const f3 g0 = vm::normalize(sub3(q1, p0));
-                    const f3 g1 = vm::normalize(sub3(q1, p1));
-                    const f3 g2 = vm::normalize(sub3(q1, p2));
-                    from_normal_edges(P, vm::normalize(vm::cross(g1, e0)), e0, g1, g0, b);
-                    from_normal_edges(P, vm::normalize(vm::cross(g2, e1)), e1, g2, g1, b);
-                    from_normal_edges(P, vm::normalize(vm::cross(g0, e2)), e2, g0, g2, b);
-                }
+                if (q0v) apex_faces(P, q0, p0, p1, p2, e0, e1, e2, pe0, pe1, pe2, sane, b);
+                if (q1v) apex_faces(P, q1, p0, p1, p2, e0, e1, e2, pe0, pe1, pe2, sane, b);
             }
         }
     }
@@ -476,12 +774,17 @@ __global__ __launch_bounds__(256) void candidates(const kr_vec3* __restrict__ V,
 //
 // Exact projection ranges of every subtree vertex on each internal node's
 // candidate axes (transforms.cu:836-900).  A workgroup takes 1024
-// consecutive leaves; each lane walks from its leaf to the root and folds its
-// triangle's min/max on every ancestor's axes into an LDS hash table keyed by
-// the ancestor (LDS integer min/max on order-preserving keys).  Consecutive
-// leaves share most ancestors, so each workgroup then issues one filtered
-// global atomic per distinct ancestor instead of one per (leaf, ancestor).
-// If the table fills, the lane falls back to global atomics for that pair.
+// consecutive leaves.  Each lane walks from its leaf up to the lowest
+// ancestor that holds all 64 leaves of its wave (a node whose leaf count
+// equals the width of its leaf span and whose span covers the wave's
+// positions; every lane of the wave stops at the same node) and folds its
+// triangle's min/max on each ancestor's axes into an LDS hash table keyed by
+// the ancestor (LDS integer min/max on order-preserving keys).  From that
+// common node to the root the wave walks together: each lane projects its
+// triangle, the wave reduces the six keys with DPP and one lane folds them
+// into the table.  The table is then folded into global memory with one
+// filtered atomic per distinct ancestor.  If the table fills, the lane falls
+// back to global atomics for that pair.
 
 constexpr int kRefitLeaves = 1024;
 constexpr int kHashSlots = 2048;  // power of two
@@ -495,13 +798,47 @@ __device__ __forceinline__ void fold_global(int32_t* rk, const int32_t v[6]) {
     }
 }
 
-__global__ __launch_bounds__(1024) void refit(const kr_bvh_node* __restrict__ nodes,
-                                              const uint32_t* __restrict__ parents,
-                                              const kr_vec3* __restrict__ V,
-                                              const kr_uvec4* __restrict__ F, TreeWs ws, uint32_t N) {
+__device__ __forceinline__ void fold_hash(uint32_t* hkey, int32_t (*hval)[6], int32_t* range, uint32_t p,
+                                          const int32_t k[6]) {
+    uint32_t h = (p * 2654435761u) >> (32 - 11);
+    int slot = -1;
+    for (int probe = 0; probe < 64; ++probe) {  // linear probing
+        uint32_t key = hkey[h];
+        if (key == kEmpty) key = atomicCAS(&hkey[h], kEmpty, p);
+        if (key == kEmpty || key == p) { slot = (int)h; break; }
+        h = (h + 1) & (kHashSlots - 1);
+    }
+    if (slot >= 0) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            atomicMin(&hval[slot][a], k[a]);
+            atomicMax(&hval[slot][3 + a], k[3 + a]);
+        }
+    } else {
+        fold_global(&range[(size_t)p * 6], k);
+    }
+}
+
+__device__ __forceinline__ void tri_keys(const Cand& c, f3 v0, f3 v1, f3 v2, int32_t k[6]) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const f3 b = f3{c.b[3 * a], c.b[3 * a + 1], c.b[3 * a + 2]};
+        const float d0 = dot3(v0, b), d1 = dot3(v1, b), d2 = dot3(v2, b);
+        k[a] = f2ord(kmin(kmin(d0, d1), d2));
+        k[3 + a] = f2ord(kmax(kmax(d0, d1), d2));
+    }
+}
+
+__global__ __launch_bounds__(kRefitLeaves) void refit(const kr_bvh_node* __restrict__ nodes,
+                                                      const uint32_t* __restrict__ parents,
+                                                      const kr_vec3* __restrict__ V,
+                                                      const kr_uvec4* __restrict__ F, TreeWs ws, uint32_t N) {
     __shared__ uint32_t hkey[kHashSlots];
     __shared__ int32_t hval[kHashSlots][6];
-    if (*ws.error != 0) return;
+    __shared__ uint32_t bad;
+    if (threadIdx.x == 0) bad = *ws.error;  // one read per workgroup: the exit is uniform
+    __syncthreads();
+    if (bad != 0) return;
     for (int h = threadIdx.x; h < kHashSlots; h += blockDim.x) {
         hkey[h] = kEmpty;
 #pragma unroll
@@ -510,42 +847,56 @@ __global__ __launch_bounds__(1024) void refit(const kr_bvh_node* __restrict__ no
     __syncthreads();
     const uint32_t I = N - 1;
     const uint64_t pos = (uint64_t)blockIdx.x * kRefitLeaves + threadIdx.x;
-    if (pos < N) {
-        const uint32_t cur = I + (uint32_t)pos;
-        const kr_uvec4 face = F[nodes[cur].u.primitivesOffset];
-        const f3 v0 = ldv(V, face.x), v1 = ldv(V, face.y), v2 = ldv(V, face.z);
-        uint32_t p = parents[cur];
-        for (uint32_t step = 0; p != kInvalid && p < I && step < N; ++step) {
-            const Cand c = ws.cand[p];
-            if (!c.degen) {
-                int32_t k[6];
+    const uint64_t wlo = pos - (uint64_t)lane_id();
+    if (wlo < N) {  // wave-uniform
+        const uint32_t whi = (uint32_t)min<uint64_t>(wlo + kWave - 1, N - 1);
+        const bool valid = pos < N;
+        f3 v0 = f3{0, 0, 0}, v1 = v0, v2 = v0;
+        uint32_t common = kInvalid;
+        if (valid) {
+            const uint32_t cur = I + (uint32_t)pos;
+            const kr_uvec4 face = F[nodes[cur].u.primitivesOffset];
+            v0 = ldv(V, face.x);
+            v1 = ldv(V, face.y);
+            v2 = ldv(V, face.z);
+            uint32_t p = parents[cur];
+            for (uint32_t step = 0; p != kInvalid && p < I && step < N; ++step) {
+                const uint2 sp = ws.span[p];
+                if (sp.x <= wlo && sp.y >= whi && !(sp.y & kGap)) { common = p; break; }
+                const Cand c = ws.cand[p];
+                if (!c.degen) {
+                    int32_t k[6];
+                    tri_keys(c, v0, v1, v2, k);
+                    fold_hash(hkey, hval, ws.range, p, k);
+                }
+                p = parents[p];
+            }
+        }
+        // lane 0 is valid (wlo < N); a well-formed tree gives every lane the same node
+        const uint32_t c0 = __builtin_amdgcn_readfirstlane(common);
+        if (__ballot(valid && common != c0) != 0) {
+            if (lane_id() == 0) atomicOr(ws.error, 2u);
+        } else {
+            uint32_t q = c0;
+            for (uint32_t step = 0; q != kInvalid && q < I && step < N; ++step) {  // uniform
+                const Cand c = ws.cand[q];
+                if (!c.degen) {
+                    int32_t k[6];
+                    if (valid) {
+                        tri_keys(c, v0, v1, v2, k);
+                    } else {
 #pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    const f3 b = f3{c.b[3 * a], c.b[3 * a + 1], c.b[3 * a + 2]};
-                    const float d0 = dot3(v0, b), d1 = dot3(v1, b), d2 = dot3(v2, b);
-                    k[a] = f2ord(kmin(kmin(d0, d1), d2));
-                    k[3 + a] = f2ord(kmax(kmax(d0, d1), d2));
-                }
-                // find / claim the slot of p (linear probing)
-                uint32_t h = (p * 2654435761u) >> (32 - 11);
-                int slot = -1;
-                for (int probe = 0; probe < 64; ++probe) {
-                    uint32_t key = hkey[h];
-                    if (key == kEmpty) key = atomicCAS(&hkey[h], kEmpty, p);
-                    if (key == kEmpty || key == p) { slot = (int)h; break; }
-                    h = (h + 1) & (kHashSlots - 1);
-                }
-                if (slot >= 0) {
+                        for (int a = 0; a < 3; ++a) { k[a] = 0x7FFFFFFF; k[3 + a] = (int32_t)0x80000000; }
+                    }
 #pragma unroll
                     for (int a = 0; a < 3; ++a) {
-                        atomicMin(&hval[slot][a], k[a]);
-                        atomicMax(&hval[slot][3 + a], k[3 + a]);
+                        k[a] = wave_min_i32(k[a]);
+                        k[3 + a] = wave_max_i32(k[3 + a]);
                     }
-                } else {
-                    fold_global(&ws.range[(size_t)p * 6], k);
+                    if (lane_id() == 0) fold_hash(hkey, hval, ws.range, q, k);
                 }
+                q = __builtin_amdgcn_readfirstlane(parents[q]);
             }
-            p = parents[p];
         }
     }
     __syncthreads();
@@ -560,68 +911,79 @@ __global__ __launch_bounds__(1024) void refit(const kr_bvh_node* __restrict__ no
 }
 
 // --------------------------------------------------------- finalize --
+//
+// One lane per internal node (transforms.cu:905-964): OBB vs AABB by half
+// area, extent floor, inverse(T * R * S).  Boxes and matrices are staged in
+// LDS (odd row strides, no bank conflicts) and written back as contiguous
+// dwords.
 
-__global__ __launch_bounds__(256) void finalize(TreeWs ws, kr_obb3* obbs, kr_mat4* mats, uint32_t I) {
-    const uint32_t node = blockIdx.x * blockDim.x + threadIdx.x;
-    if (node >= I) return;
-    if (*ws.error != 0 || !complete(ws.visit[node])) {
-        const float nan = __builtin_nanf("");
-        store_obb(&obbs[node], f3{nan, nan, nan}, f3{nan, nan, nan}, f3{nan, nan, nan}, f3{nan, nan, nan},
-                  f3{nan, nan, nan});
-        float M[16];
+__global__ __launch_bounds__(kStage) void finalize(TreeWs ws, kr_obb3* obbs, kr_mat4* mats, uint32_t I) {
+    __shared__ Stage st;
+    const uint32_t base = blockIdx.x * kStage;
+    const uint32_t node = base + threadIdx.x;
+    float* ob = &st.ob[threadIdx.x * 15];
+    float* mt = &st.mt[threadIdx.x * 17];
+    if (node < I) {
+        if (*ws.error != 0 || !complete(ws.visit[node])) {
+            stage_nan(st, threadIdx.x);
+        } else {
+            const float4* kd = &ws.kdop[(size_t)node * 7];
+            const float4 s0 = kd[0], s1 = kd[1], s2 = kd[2];
+            const f3 amn = f3{s0.x, s1.x, s2.x}, amx = f3{s0.z, s1.z, s2.z};
+            const f3 alen = sub3(amx, amn);
+            const Cand c = ws.cand[node];
+            const int32_t* rk = &ws.range[(size_t)node * 6];
+            const f3 rmn = f3{ord2f(rk[0]), ord2f(rk[1]), ord2f(rk[2])};
+            const f3 rmx = f3{ord2f(rk[3]), ord2f(rk[4]), ord2f(rk[5])};
+            f3 v0, v1, v2, mn, mx;
+            if (!c.degen && quality(sub3(rmx, rmn)) < quality(alen)) {
+                v0 = f3{c.b[0], c.b[1], c.b[2]};
+                v1 = f3{c.b[3], c.b[4], c.b[5]};
+                v2 = f3{c.b[6], c.b[7], c.b[8]};
+                mn = rmn;
+                mx = rmx;
+            } else {
+                v0 = f3{1, 0, 0};
+                v1 = f3{0, 1, 0};
+                v2 = f3{0, 0, 1};
+                mn = amn;
+                mx = amx;
+            }
+            const f3 mid = vm::mul1(add3(mn, mx), 0.5f);
+            const f3 ext = vm::vmax3(vm::mul1(sub3(mx, mn), 0.5f), f3{0.001f, 0.001f, 0.001f});
+            f3 m = vm::mul1(v0, mid.x);
+            m = add3(m, vm::mul1(v1, mid.y));
+            m = add3(m, vm::mul1(v2, mid.z));
+            obb_row(ob, m, v0, v1, v2, ext);
+            float M[16];
+            obb_matrix(v0, v1, v2, m, ext, M);
 #pragma unroll
-        for (int q = 0; q < 16; ++q) M[q] = nan;
-        store_mat(&mats[node], M);
-        return;
+            for (int q = 0; q < 16; ++q) mt[q] = M[q];
+        }
     }
-    const float4* kd = &ws.kdop[(size_t)node * 7];
-    const float4 s0 = kd[0], s1 = kd[1], s2 = kd[2];
-    const f3 amn = f3{s0.x, s1.x, s2.x}, amx = f3{s0.z, s1.z, s2.z};
-    const f3 alen = sub3(amx, amn);
-    const Cand c = ws.cand[node];
-    const int32_t* rk = &ws.range[(size_t)node * 6];
-    const f3 rmn = f3{ord2f(rk[0]), ord2f(rk[1]), ord2f(rk[2])};
-    const f3 rmx = f3{ord2f(rk[3]), ord2f(rk[4]), ord2f(rk[5])};
-    f3 v0, v1, v2, mn, mx;
-    if (!c.degen && quality(sub3(rmx, rmn)) < quality(alen)) {
-        v0 = f3{c.b[0], c.b[1], c.b[2]};
-        v1 = f3{c.b[3], c.b[4], c.b[5]};
-        v2 = f3{c.b[6], c.b[7], c.b[8]};
-        mn = rmn;
-        mx = rmx;
-    } else {
-        v0 = f3{1, 0, 0};
-        v1 = f3{0, 1, 0};
-        v2 = f3{0, 0, 1};
-        mn = amn;
-        mx = amx;
-    }
-    const f3 mid = vm::mul1(add3(mn, mx), 0.5f);
-    const f3 ext = vm::vmax3(vm::mul1(sub3(mx, mn), 0.5f), f3{0.001f, 0.001f, 0.001f});
-    f3 m = vm::mul1(v0, mid.x);
-    m = add3(m, vm::mul1(v1, mid.y));
-    m = add3(m, vm::mul1(v2, mid.z));
-    store_obb(&obbs[node], m, v0, v1, v2, ext);
-    float M[16];
-    obb_matrix(v0, v1, v2, m, ext, M);
-    store_mat(&mats[node], M);
+    __syncthreads();
+    stage_flush(st, obbs, mats, base, min((uint32_t)kStage, I - min(base, I)));
 }
 
-__global__ void copy_ids(TreeWs ws, uint32_t* ids, uint32_t total) {
+__global__ void copy_ids(const kr_bvh_node* __restrict__ nodes, const kr_vec3* __restrict__ V,
+                         const kr_uvec4* __restrict__ F, TreeWs ws, uint32_t* ids, uint32_t total) {
     const uint32_t node = blockIdx.x * blockDim.x + threadIdx.x;
     if (node >= total) return;
-    const uint32_t I = (total - 1) / 2;
-    if (*ws.error != 0 || (node < I && !complete(ws.visit[node]))) {
-#pragma unroll
-        for (int s = 0; s < 14; ++s) ids[(size_t)node * 14 + s] = 0xFFFFFFFFu;
-        return;
+    const uint32_t I = (total - 1) / 2, N = I + 1;
+    Kdop k;
+    bool ok = *ws.error == 0;
+    if (ok) {
+        if (node < I) {
+            ok = complete(ws.visit[node]);
+            if (ok) kdop_load(ws.kdop, node, k);
+        } else {
+            ok = leaf_kdop_global(nodes, V, F, node, N, k);
+        }
     }
-    const float4* kd = &ws.kdop[(size_t)node * 7];
 #pragma unroll
     for (int s = 0; s < 7; ++s) {
-        float4 a = kd[s];
-        ids[(size_t)node * 14 + s] = __float_as_uint(a.y);
-        ids[(size_t)node * 14 + 7 + s] = __float_as_uint(a.w);
+        ids[(size_t)node * 14 + s] = ok ? k.imn[s] : 0xFFFFFFFFu;
+        ids[(size_t)node * 14 + 7 + s] = ok ? k.imx[s] : 0xFFFFFFFFu;
     }
 }
 
@@ -632,7 +994,6 @@ size_t tree_carve(TreeWs* ws, void* base, size_t cap, uint32_t N) {
     const size_t I = N ? N - 1 : 0, total = N ? 2 * (size_t)N - 1 : 0;
     ws->visit = c.take<uint64_t>(I + 1);
     ws->error = c.take<uint32_t>(8);
-    ws->vmax = ws->error ? ws->error + 1 : nullptr;
     ws->nlist = ws->error ? ws->error + 2 : nullptr;
     ws->list = c.take<uint32_t>((size_t)kMaxPhases * (I + 1));
     ws->span = c.take<uint2>(total);
@@ -652,22 +1013,27 @@ kr_error tree_launch(const kr_bvh_node* nodes, const uint32_t* parents, const kr
     const uint32_t I = N - 1;
     auto blocks = [](uint32_t n, uint32_t b) { return dim3((n + b - 1) / b); };
     hipLaunchKernelGGL(tree_clear, dim3(1024), dim3(256), 0, stream, ws.visit, I + 1, ws);
-    hipLaunchKernelGGL(leaf_boxes, blocks(N, 256), dim3(256), 0, stream, nodes, V, F, obbs, mats, ws, N);
-    // climb phases: 1024-leaf workgroups, then 32768-leaf workgroups, then one
-    uint32_t phase = 0;
-    for (uint32_t chunk : {kChunk0, kChunk1, 0u}) {
-        const bool last = (chunk == 0) || chunk >= N;
-        const uint32_t c = last ? N : chunk;
-        hipLaunchKernelGGL(climb_phase, blocks(N, c), dim3(1024), 0, stream, nodes, parents, V, F, ws, N, phase, c);
-        ++phase;
-        if (last) break;
+    hipLaunchKernelGGL(leaf_boxes, blocks(N, kStage), dim3(kStage), 0, stream, nodes, V, F, obbs, mats, N);
+    // climb: 1024-leaf tiles, then 32768-leaf workgroups, then one workgroup
+    // for the rest
+    hipLaunchKernelGGL(climb_leaves, blocks(N, kTile), dim3(kTile), 0, stream, nodes, parents, V, F, ws, N);
+    if (N > kTile) {
+        uint32_t phase = 1;
+        for (uint32_t chunk : {kChunk1, 0u}) {
+            const bool last = (chunk == 0) || chunk >= N;
+            const uint32_t c = last ? N : chunk;
+            hipLaunchKernelGGL(climb_phase, blocks(N, c), dim3(1024), 0, stream, nodes, parents, ws, N, phase, c);
+            ++phase;
+            if (last) break;
+        }
     }
     if (I > 0) {
         hipLaunchKernelGGL(candidates, blocks(I, 256), dim3(256), 0, stream, V, ws, I);
         hipLaunchKernelGGL(refit, blocks(N, kRefitLeaves), dim3(kRefitLeaves), 0, stream, nodes, parents, V, F, ws, N);
-        hipLaunchKernelGGL(finalize, blocks(I, 256), dim3(256), 0, stream, ws, obbs, mats, I);
+        hipLaunchKernelGGL(finalize, blocks(I, kStage), dim3(kStage), 0, stream, ws, obbs, mats, I);
     }
-    if (ids) hipLaunchKernelGGL(copy_ids, blocks(2 * N - 1, 256), dim3(256), 0, stream, ws, ids, 2 * N - 1);
+    if (ids)
+        hipLaunchKernelGGL(copy_ids, blocks(2 * N - 1, 256), dim3(256), 0, stream, nodes, V, F, ws, ids, 2 * N - 1);
     OBVH_HIP_TRY(hipGetLastError());
     return nullptr;
 }

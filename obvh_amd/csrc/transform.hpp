@@ -17,10 +17,9 @@ constexpr int kMaxPhases = 4;
 
 struct TreeWs {
     uint64_t* visit;   // per internal node: arrivals (low word) + sum of arrival tags (high word)
-    uint32_t* vmax;    // largest vertex id referenced by a face
     uint32_t* nlist;   // per climb phase: number of deferred nodes
     uint32_t* list;    // per climb phase: nodes deferred to the next phase
-    uint2* span;       // per node: first and last leaf position below it
+    uint2* span;       // per node: first and last leaf position below it (bit 31 of .y: not contiguous)
     uint32_t* error;   // 1: parent index out of range / too deep, 2: parent/child mismatch,
                        // 4: face id out of range, 8: a node did not complete
     float4* kdop;      // 7 per node: {min, min vertex id, max, max vertex id}
