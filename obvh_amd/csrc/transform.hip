@@ -772,23 +772,48 @@ __global__ __launch_bounds__(256, 4) void candidates(const kr_vec3* __restrict__
 
 // ------------------------------------------------------------ refit --
 //
-// Exact projection ranges of every subtree vertex on each internal node's
-// candidate axes (transforms.cu:836-900).  A workgroup takes 1024
-// consecutive leaves.  Each lane walks from its leaf up to the lowest
-// ancestor that holds all 64 leaves of its wave (a node whose leaf count
-// equals the width of its leaf span and whose span covers the wave's
-// positions; every lane of the wave stops at the same node) and folds its
-// triangle's min/max on each ancestor's axes into an LDS hash table keyed by
-// the ancestor (LDS integer min/max on order-preserving keys).  From that
-// common node to the root the wave walks together: each lane projects its
-// triangle, the wave reduces the six keys with DPP and one lane folds them
-// into the table.  The table is then folded into global memory with one
-// filtered atomic per distinct ancestor.  If the table fills, the lane falls
-// back to global atomics for that pair.
+// Exact projection range of every subtree vertex on each internal node's
+// candidate axes (transforms.cu:836-900), as order-preserving int32 keys
+// (-0 orders below +0), min/max over the subtree's vertices.  A workgroup
+// takes a tile of 1024 consecutive leaves and splits each leaf's ancestors
+// in three tiers:
+//   A  below the lowest ancestor holding all 64 leaves of the lane's wave:
+//      each lane folds its triangle's keys into LDS (a slot per internal
+//      node of the tile's index range, a small hash for other nodes);
+//   B  from that node up to the lowest ancestor holding the whole tile: the
+//      wave walks together, reduces the 64 lanes' keys with DPP and one lane
+//      folds them into LDS;
+//   C  the tile's common ancestor and everything above it: the tile's
+//      distinct vertices are collected once (LDS hash set on vertex ids) and
+//      each wave takes some of those ancestors, looping over the distinct
+//      vertices instead of 3 per triangle.
+// "Holds all leaves of a range" is read from the span: its first and last
+// leaf cover the range and the leaves below are contiguous.  LDS results are
+// folded into global memory with filtered atomics at the end.  Without the
+// tile-wide pass (too many distinct vertices, or no tile-common node) tier B
+// continues to the root.
 
-constexpr int kRefitLeaves = 1024;
-constexpr int kHashSlots = 2048;  // power of two
+constexpr int kRHash = 256;      // LDS hash for ancestors outside the tile's index range
+constexpr int kVSet = 2048;      // distinct vertex ids per tile (hash set)
+constexpr int kChainMax = 128;   // ancestors handled by the tile-wide pass per batch
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+constexpr int32_t kKeyHi = 0x7FFFFFFF, kKeyLo = (int32_t)0x80000000;
+
+struct RefitTile {
+    int32_t tk[kTile][6];        // keys of internal node lo + s: min b0..b2, max b0..b2
+    uint32_t hkey[kRHash];
+    int32_t hval[kRHash][6];
+    uint32_t vkey[kVSet];        // distinct vertex ids of the tile's triangles
+    float upos[kVSet][3];        // their positions, compacted
+    uint32_t chain[kChainMax];   // tile-common node and its ancestors
+    uint32_t nchain, nuniq, overflow, rest;
+};
+
+// Order-preserving key of a float (same map as f2ord, without a branch).
+__device__ __forceinline__ int32_t okey(float f) {
+    const int32_t i = __float_as_int(f);
+    return i ^ (int32_t)((uint32_t)(i >> 31) >> 1);
+}
 
 __device__ __forceinline__ void fold_global(int32_t* rk, const int32_t v[6]) {
 #pragma unroll
@@ -798,86 +823,200 @@ __device__ __forceinline__ void fold_global(int32_t* rk, const int32_t v[6]) {
     }
 }
 
-__device__ __forceinline__ void fold_hash(uint32_t* hkey, int32_t (*hval)[6], int32_t* range, uint32_t p,
-                                          const int32_t k[6]) {
-    uint32_t h = (p * 2654435761u) >> (32 - 11);
-    int slot = -1;
-    for (int probe = 0; probe < 64; ++probe) {  // linear probing
-        uint32_t key = hkey[h];
-        if (key == kEmpty) key = atomicCAS(&hkey[h], kEmpty, p);
-        if (key == kEmpty || key == p) { slot = (int)h; break; }
-        h = (h + 1) & (kHashSlots - 1);
-    }
-    if (slot >= 0) {
+__device__ __forceinline__ void fold_lds(int32_t* slot, const int32_t v[6]) {
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            atomicMin(&hval[slot][a], k[a]);
-            atomicMax(&hval[slot][3 + a], k[3 + a]);
-        }
-    } else {
-        fold_global(&range[(size_t)p * 6], k);
+    for (int a = 0; a < 3; ++a) {
+        if (v[a] < slot[a]) atomicMin(&slot[a], v[a]);
+        if (v[3 + a] > slot[3 + a]) atomicMax(&slot[3 + a], v[3 + a]);
     }
+}
+
+__device__ __forceinline__ void fold_tile(RefitTile& R, int32_t* range, uint32_t lo, uint32_t p, const int32_t k[6]) {
+    if (p - lo < kTile) {
+        fold_lds(R.tk[p - lo], k);
+        return;
+    }
+    uint32_t h = (p * 2654435761u) >> 24;
+    for (int probe = 0; probe < 16; ++probe) {  // linear probing
+        uint32_t key = R.hkey[h];
+        if (key == kEmpty) key = atomicCAS(&R.hkey[h], kEmpty, p);
+        if (key == kEmpty || key == p) {
+            fold_lds(R.hval[h], k);
+            return;
+        }
+        h = (h + 1) & (kRHash - 1);
+    }
+    fold_global(&range[(size_t)p * 6], k);
 }
 
 __device__ __forceinline__ void tri_keys(const Cand& c, f3 v0, f3 v1, f3 v2, int32_t k[6]) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         const f3 b = f3{c.b[3 * a], c.b[3 * a + 1], c.b[3 * a + 2]};
-        const float d0 = dot3(v0, b), d1 = dot3(v1, b), d2 = dot3(v2, b);
-        k[a] = f2ord(kmin(kmin(d0, d1), d2));
-        k[3 + a] = f2ord(kmax(kmax(d0, d1), d2));
+        const int32_t k0 = okey(dot3(v0, b)), k1 = okey(dot3(v1, b)), k2 = okey(dot3(v2, b));
+        k[a] = min(min(k0, k1), k2);
+        k[3 + a] = max(max(k0, k1), k2);
     }
 }
 
-__global__ __launch_bounds__(kRefitLeaves) void refit(const kr_bvh_node* __restrict__ nodes,
-                                                      const uint32_t* __restrict__ parents,
-                                                      const kr_vec3* __restrict__ V,
-                                                      const kr_uvec4* __restrict__ F, TreeWs ws, uint32_t N) {
-    __shared__ uint32_t hkey[kHashSlots];
-    __shared__ int32_t hval[kHashSlots][6];
-    __shared__ uint32_t bad;
-    if (threadIdx.x == 0) bad = *ws.error;  // one read per workgroup: the exit is uniform
-    __syncthreads();
-    if (bad != 0) return;
-    for (int h = threadIdx.x; h < kHashSlots; h += blockDim.x) {
-        hkey[h] = kEmpty;
+__device__ __forceinline__ void wave_keys(int32_t k[6]) {
 #pragma unroll
-        for (int a = 0; a < 3; ++a) { hval[h][a] = 0x7FFFFFFF; hval[h][3 + a] = (int32_t)0x80000000; }
+    for (int a = 0; a < 3; ++a) {
+        k[a] = wave_min_i32(k[a]);
+        k[3 + a] = wave_max_i32(k[3 + a]);
+    }
+}
+
+__device__ __forceinline__ bool holds(uint2 sp, uint32_t first, uint32_t last) {
+    return sp.x <= first && sp.y >= last && !(sp.y & kGap);
+}
+
+__global__ __launch_bounds__(kTile) void refit(const kr_bvh_node* __restrict__ nodes,
+                                               const uint32_t* __restrict__ parents,
+                                               const kr_vec3* __restrict__ V,
+                                               const kr_uvec4* __restrict__ F, TreeWs ws, uint32_t N) {
+    __shared__ RefitTile R;
+    __shared__ uint32_t bad;
+    const uint32_t I = N - 1, t = threadIdx.x;
+    const uint32_t lo = blockIdx.x * kTile;
+    const uint32_t thi = min(lo + kTile, N) - 1;  // last leaf position of the tile
+#pragma unroll
+    for (int a = 0; a < 3; ++a) { R.tk[t][a] = kKeyHi; R.tk[t][3 + a] = kKeyLo; }
+    if (t < kRHash) {
+        R.hkey[t] = kEmpty;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) { R.hval[t][a] = kKeyHi; R.hval[t][3 + a] = kKeyLo; }
+    }
+    for (uint32_t h = t; h < kVSet; h += kTile) R.vkey[h] = kEmpty;
+    if (t == 0) {
+        bad = *ws.error;  // one read per workgroup: the exit is uniform
+        R.nchain = 0;
+        R.nuniq = 0;
+        R.overflow = 0;
+        R.rest = kInvalid;
     }
     __syncthreads();
-    const uint32_t I = N - 1;
-    const uint64_t pos = (uint64_t)blockIdx.x * kRefitLeaves + threadIdx.x;
-    const uint64_t wlo = pos - (uint64_t)lane_id();
+    if (bad != 0) return;
+
+    const uint32_t pos = lo + t;
+    const uint32_t wlo = pos - (uint32_t)lane_id();
+    const uint32_t whi = min(wlo + kWave - 1, N - 1);
+    const bool valid = pos < N;
+    f3 v0 = f3{0, 0, 0}, v1 = v0, v2 = v0;
+    if (valid) {
+        const kr_uvec4 face = F[nodes[I + pos].u.primitivesOffset];
+        v0 = ldv(V, face.x);
+        v1 = ldv(V, face.y);
+        v2 = ldv(V, face.z);
+        const uint32_t vid[3] = {face.x, face.y, face.z};
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {  // distinct vertex ids of the tile
+            uint32_t h = (vid[j] * 2654435761u) >> 21;
+            bool placed = false;
+            for (int probe = 0; probe < 32 && !placed; ++probe) {
+                uint32_t key = R.vkey[h];
+                if (key == kEmpty) key = atomicCAS(&R.vkey[h], kEmpty, vid[j]);
+                placed = key == kEmpty || key == vid[j];
+                h = (h + 1) & (kVSet - 1);
+            }
+            if (!placed) R.overflow = 1;
+        }
+    }
+    __syncthreads();
+    const bool tile_pass = R.overflow == 0;
+
     if (wlo < N) {  // wave-uniform
-        const uint32_t whi = (uint32_t)min<uint64_t>(wlo + kWave - 1, N - 1);
-        const bool valid = pos < N;
-        f3 v0 = f3{0, 0, 0}, v1 = v0, v2 = v0;
+        // tier A
         uint32_t common = kInvalid;
         if (valid) {
-            const uint32_t cur = I + (uint32_t)pos;
-            const kr_uvec4 face = F[nodes[cur].u.primitivesOffset];
-            v0 = ldv(V, face.x);
-            v1 = ldv(V, face.y);
-            v2 = ldv(V, face.z);
-            uint32_t p = parents[cur];
+            uint32_t p = parents[I + pos];
             for (uint32_t step = 0; p != kInvalid && p < I && step < N; ++step) {
-                const uint2 sp = ws.span[p];
-                if (sp.x <= wlo && sp.y >= whi && !(sp.y & kGap)) { common = p; break; }
+                if (holds(ws.span[p], wlo, whi)) { common = p; break; }
                 const Cand c = ws.cand[p];
                 if (!c.degen) {
                     int32_t k[6];
                     tri_keys(c, v0, v1, v2, k);
-                    fold_hash(hkey, hval, ws.range, p, k);
+                    fold_tile(R, ws.range, lo, p, k);
                 }
                 p = parents[p];
             }
         }
-        // lane 0 is valid (wlo < N); a well-formed tree gives every lane the same node
+        // tier B; lane 0 is valid (wlo < N) and a well-formed tree gives
+        // every valid lane the same node
         const uint32_t c0 = __builtin_amdgcn_readfirstlane(common);
         if (__ballot(valid && common != c0) != 0) {
             if (lane_id() == 0) atomicOr(ws.error, 2u);
         } else {
             uint32_t q = c0;
+            for (uint32_t step = 0; q != kInvalid && q < I && step < N; ++step) {  // uniform
+                if (tile_pass && holds(ws.span[q], lo, thi)) break;  // tier C takes q and above
+                const Cand c = ws.cand[q];
+                if (!c.degen) {
+                    int32_t k[6];
+                    if (valid) {
+                        tri_keys(c, v0, v1, v2, k);
+                    } else {
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) { k[a] = kKeyHi; k[3 + a] = kKeyLo; }
+                    }
+                    wave_keys(k);
+                    if (lane_id() == 0) fold_tile(R, ws.range, lo, q, k);
+                }
+                q = __builtin_amdgcn_readfirstlane(parents[q]);
+            }
+            // wave 0 stopped at the tile-common node: record it and its ancestors
+            if (tile_pass && t < kWave && q != kInvalid && q < I) {
+                uint32_t n = 0;
+                while (q != kInvalid && q < I && n < (uint32_t)kChainMax) {
+                    if (lane_id() == 0) R.chain[n] = q;
+                    ++n;
+                    q = __builtin_amdgcn_readfirstlane(parents[q]);
+                }
+                if (lane_id() == 0) {
+                    R.nchain = n;
+                    R.rest = (q != kInvalid && q < I) ? q : kInvalid;
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    if (tile_pass) {
+        for (uint32_t h = t; h < kVSet; h += kTile) {
+            const uint32_t id = R.vkey[h];
+            if (id == kEmpty) continue;
+            const uint32_t u = atomicAdd(&R.nuniq, 1u);
+            const f3 p = ldv(V, id);
+            R.upos[u][0] = p.x;
+            R.upos[u][1] = p.y;
+            R.upos[u][2] = p.z;
+        }
+        __syncthreads();
+        // tier C over the distinct vertices
+        const uint32_t nu = R.nuniq, nc = R.nchain;
+        for (uint32_t j = (uint32_t)wave_id(); j < nc; j += kTile / kWave) {
+            const uint32_t q = __builtin_amdgcn_readfirstlane(R.chain[j]);
+            const Cand c = ws.cand[q];
+            if (c.degen) continue;
+            int32_t k[6];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) { k[a] = kKeyHi; k[3 + a] = kKeyLo; }
+            for (uint32_t u = (uint32_t)lane_id(); u < nu; u += kWave) {
+                const f3 p = f3{R.upos[u][0], R.upos[u][1], R.upos[u][2]};
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const int32_t kk = okey(dot3(p, f3{c.b[3 * a], c.b[3 * a + 1], c.b[3 * a + 2]}));
+                    k[a] = min(k[a], kk);
+                    k[3 + a] = max(k[3 + a], kk);
+                }
+            }
+            wave_keys(k);
+            if (lane_id() == 0) fold_global(&ws.range[(size_t)q * 6], k);
+        }
+        // ancestors beyond the recorded batch (very deep trees): every wave
+        // walks them with its lanes' triangles
+        uint32_t q = R.rest;
+        if (wlo < N) {
             for (uint32_t step = 0; q != kInvalid && q < I && step < N; ++step) {  // uniform
                 const Cand c = ws.cand[q];
                 if (!c.degen) {
@@ -886,28 +1025,20 @@ __global__ __launch_bounds__(kRefitLeaves) void refit(const kr_bvh_node* __restr
                         tri_keys(c, v0, v1, v2, k);
                     } else {
 #pragma unroll
-                        for (int a = 0; a < 3; ++a) { k[a] = 0x7FFFFFFF; k[3 + a] = (int32_t)0x80000000; }
+                        for (int a = 0; a < 3; ++a) { k[a] = kKeyHi; k[3 + a] = kKeyLo; }
                     }
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) {
-                        k[a] = wave_min_i32(k[a]);
-                        k[3 + a] = wave_max_i32(k[3 + a]);
-                    }
-                    if (lane_id() == 0) fold_hash(hkey, hval, ws.range, q, k);
+                    wave_keys(k);
+                    if (lane_id() == 0) fold_global(&ws.range[(size_t)q * 6], k);
                 }
                 q = __builtin_amdgcn_readfirstlane(parents[q]);
             }
         }
     }
     __syncthreads();
-    for (int h = threadIdx.x; h < kHashSlots; h += blockDim.x) {
-        const uint32_t node = hkey[h];
-        if (node == kEmpty) continue;
-        int32_t k[6];
-#pragma unroll
-        for (int a = 0; a < 6; ++a) k[a] = hval[h][a];
-        fold_global(&ws.range[(size_t)node * 6], k);
-    }
+    // fold the tile's LDS keys into global memory
+    if (lo + t < I && (R.tk[t][0] != kKeyHi || R.tk[t][3] != kKeyLo))
+        fold_global(&ws.range[(size_t)(lo + t) * 6], R.tk[t]);
+    if (t < kRHash && R.hkey[t] != kEmpty) fold_global(&ws.range[(size_t)R.hkey[t] * 6], R.hval[t]);
 }
 
 // --------------------------------------------------------- finalize --
@@ -1029,7 +1160,7 @@ kr_error tree_launch(const kr_bvh_node* nodes, const uint32_t* parents, const kr
     }
     if (I > 0) {
         hipLaunchKernelGGL(candidates, blocks(I, 256), dim3(256), 0, stream, V, ws, I);
-        hipLaunchKernelGGL(refit, blocks(N, kRefitLeaves), dim3(kRefitLeaves), 0, stream, nodes, parents, V, F, ws, N);
+        hipLaunchKernelGGL(refit, blocks(N, kTile), dim3(kTile), 0, stream, nodes, parents, V, F, ws, N);
         hipLaunchKernelGGL(finalize, blocks(I, kStage), dim3(kStage), 0, stream, ws, obbs, mats, I);
     }
     if (ids)
