@@ -190,6 +190,67 @@ def serving_leg(args) -> dict:
     return out
 
 
+def transform_leg(V, F, label: str, steps: int, warmup: int, with_cpu: bool) -> dict:
+    """LBVH of the mesh built on the GPU, then `steps` timed transforms of the
+    resident tree; optionally the CPU oracle on the same tree (bit-exact
+    comparison and single-thread CPU throughput)."""
+    import numpy as np
+    import torch
+
+    from obvh_amd import ops
+
+    N, NV = F.shape[0], V.shape[0]
+    Vd = torch.from_numpy(V).cuda()
+    Fd = torch.from_numpy(F.view(np.int32)).cuda()
+    tree = ops.lbvh_build(Vd, Fd)
+    total = 2 * N - 1
+    obbs = torch.empty((total, 15), dtype=torch.float32, device="cuda")
+    mats = torch.empty((total, 16), dtype=torch.float32, device="cuda")
+    ws = torch.empty(ops.lib().obvh_obb_tree_workspace_size(N, N - 1), dtype=torch.uint8, device="cuda")
+
+    def step():
+        ops.obb_tree(tree["nodes"], tree["parents"], Vd, Fd, workspace=ws, out=(obbs, mats))
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    t0 = time.perf_counter()
+    for k in range(steps):
+        evs[k][0].record()
+        step()
+        evs[k][1].record()
+    torch.cuda.synchronize()
+    ms_step = (time.perf_counter() - t0) * 1e3 / steps
+    dev_ms = sorted(a.elapsed_time(b) for a, b in evs)
+    dev_ms_mean = sum(dev_ms) / len(dev_ms)
+    internal = N - 1
+    value = internal / (ms_step * 1e-3) / 1e6
+    balg = b_alg_transform(N, NV)
+    achieved = balg / (dev_ms_mean * 1e-3) / 1e9
+    print(f"[bench] transform {label}: {N} triangles, {ms_step:.4f} ms/step wall, {dev_ms_mean:.4f} ms device "
+          f"(min {dev_ms[0]:.4f}), {value:.1f} Mnodes/s, {achieved:.1f} GB/s", file=sys.stderr, flush=True)
+    out = {"triangles": N, "vertices": NV, "internal_nodes": internal, "steps": steps,
+           "ms_per_step": round(ms_step, 5), "device_ms": round(dev_ms_mean, 5),
+           "Mnodes_per_s": round(value, 2), "bytes_per_call": balg, "achieved_GBs": round(achieved, 1),
+           "roofline_frac": round(achieved / HBM_PEAK_GBS, 4), "cpu_baseline": None, "parity": None}
+    if with_cpu:
+        import oracle
+
+        nodes_h = ops.nodes_to_numpy(tree["nodes"])
+        parents_h = tree["parents"].cpu().numpy().view(np.uint32)
+        tc = time.perf_counter()
+        eo, em = oracle.obb_tree(nodes_h, parents_h, V, F)
+        cpu_s = time.perf_counter() - tc
+        out["parity"] = bool(np.array_equal(obbs.cpu().numpy(), eo) and np.array_equal(mats.cpu().numpy(), em))
+        out["cpu_baseline"] = {
+            "value": round(internal / cpu_s / 1e6, 4), "unit": "Mnodes/s", "cores": 1, "kind": "port",
+            "sample": f"whole workload: one transform of the {N}-triangle tree ({cpu_s:.2f} s, single thread)",
+        }
+        print(f"[bench] cpu oracle {label}: {cpu_s:.2f}s, parity={out['parity']}", file=sys.stderr, flush=True)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -201,6 +262,7 @@ def main() -> int:
     ap.add_argument("--sets", type=int, default=8192, help="batched DiTO: number of sets")
     ap.add_argument("--no-plain", action="store_true", help="skip the one-call-per-set DiTO comparison")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-2m", action="store_true", help="skip the 2M-triangle transform legs")
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="open-loop serving duration")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="open-loop arrival rate (req/s)")
     args = ap.parse_args()
@@ -209,71 +271,34 @@ def main() -> int:
               "sharding over several GPUs is not launched", file=sys.stderr)
         return 2
 
-    import numpy as np
     import torch
 
-    from obvh_amd import ops, synth
+    from obvh_amd import synth
 
     torch.cuda.set_device(0)
     dev = torch.cuda.get_device_name(0)
     t_gen = time.perf_counter()
     V, F = synth.icosphere(args.freq, seed=2)
-    N, NV = F.shape[0], V.shape[0]
-    Vd = torch.from_numpy(V).cuda()
-    Fd = torch.from_numpy(F.view(np.int32)).cuda()
-    tree = ops.lbvh_build(Vd, Fd)
-    torch.cuda.synchronize()
-    print(f"[bench] device={dev} triangles={N} vertices={NV} inputs ready in "
-          f"{time.perf_counter() - t_gen:.1f}s", file=sys.stderr, flush=True)
+    print(f"[bench] device={dev}", file=sys.stderr, flush=True)
+    main_leg = transform_leg(V, F, f"icosphere f={args.freq}", args.steps, args.warmup, not args.no_cpu)
+    print(f"[bench] main workload ready and timed in {time.perf_counter() - t_gen:.1f}s", file=sys.stderr, flush=True)
+    N, NV, internal = main_leg["triangles"], main_leg["vertices"], main_leg["internal_nodes"]
+    ms_step, dev_ms_mean, value, achieved = (main_leg["ms_per_step"], main_leg["device_ms"],
+                                              main_leg["Mnodes_per_s"], main_leg["achieved_GBs"])
+    balg, cpu, parity = main_leg["bytes_per_call"], main_leg["cpu_baseline"], main_leg["parity"]
 
-    total = 2 * N - 1
-    obbs = torch.empty((total, 15), dtype=torch.float32, device="cuda")
-    mats = torch.empty((total, 16), dtype=torch.float32, device="cuda")
-    ws = torch.empty(ops.lib().obvh_obb_tree_workspace_size(N, N - 1), dtype=torch.uint8, device="cuda")
-
-    def step():
-        ops.obb_tree(tree["nodes"], tree["parents"], Vd, Fd, workspace=ws, out=(obbs, mats))
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        evs[k][0].record()
-        step()
-        evs[k][1].record()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms_step = (t1 - t0) * 1e3 / args.steps
-    dev_ms = sorted(a.elapsed_time(b) for a, b in evs)
-    dev_ms_mean = sum(dev_ms) / len(dev_ms)
-    internal = N - 1
-    value = internal / (ms_step * 1e-3) / 1e6
-    balg = b_alg_transform(N, NV)
-    achieved = balg / (dev_ms_mean * 1e-3) / 1e9
-    print(f"[bench] transform: {ms_step:.4f} ms/step wall, {dev_ms_mean:.4f} ms device "
-          f"(min {dev_ms[0]:.4f}), {value:.1f} Mnodes/s, {achieved:.1f} GB/s", file=sys.stderr, flush=True)
-
-    cpu = None
-    parity = None
-    if not args.no_cpu:
-        import oracle
-
-        nodes_h = ops.nodes_to_numpy(tree["nodes"])
-        parents_h = tree["parents"].cpu().numpy().view(np.uint32)
-        tc = time.perf_counter()
-        eo, em = oracle.obb_tree(nodes_h, parents_h, V, F)
-        cpu_s = time.perf_counter() - tc
-        parity = bool(np.array_equal(obbs.cpu().numpy(), eo) and np.array_equal(mats.cpu().numpy(), em))
-        cpu = {
-            "value": round(internal / cpu_s / 1e6, 4),
-            "unit": "Mnodes/s",
-            "cores": 1,
-            "kind": "port",
-            "sample": f"whole workload: one transform of the {N}-triangle tree ({cpu_s:.2f} s, single thread)",
-        }
-        print(f"[bench] cpu oracle {cpu_s:.2f}s, parity={parity}", file=sys.stderr, flush=True)
+    extra = None
+    if not args.no_2m:
+        extra = {}
+        for name, gen in (("icosphere_f316", lambda: synth.icosphere(316, seed=2)),
+                          ("hairball_lbvh", lambda: synth.hairball(seed=3))):
+            try:
+                V2, F2 = gen()
+                extra[name] = transform_leg(V2, F2, name, max(5, args.steps // 4), 2, not args.no_cpu)
+                del V2, F2
+            except Exception as e:
+                print(f"[bench] {name} leg failed: {e!r}", file=sys.stderr, flush=True)
+                extra[name] = {"error": repr(e)}
 
     dito = None
     if not args.no_dito:
@@ -325,6 +350,7 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
         "parity": parity,
+        "transform_2m": extra,
         "dito": dito,
         "serving": served,
     }

@@ -15,6 +15,7 @@
  *   kr_vec3 / kr_uvec4 / kr_obb3 / kr_mat4   src/common/vecmath.h:136,172,156,158
  *   kr_bvh_node                              src/common/ads.h:131-142
  *   kr_error (NULL = success)                src/common/vecmath.h:73,21-24
+ *   kr_ads_collapse_metrics                  src/common/ads.h:210-214
  */
 #ifndef OBVH_OBVH_H
 #define OBVH_OBVH_H
@@ -31,6 +32,7 @@ extern "C" {
 /* ------------------------------------------------------------------------ */
 
 typedef uint32_t u32;
+typedef int32_t kr_i32;
 typedef uint64_t kr_size;
 typedef float kr_scalar;
 typedef const void* kr_error; /* NULL on success, else a static message */
@@ -54,6 +56,12 @@ typedef struct {
     uint8_t axis;
     uint8_t pad_;
 } kr_bvh_node;
+
+/* Collapse timings (ms), accumulated by kr_cuda_bvh_collapse. */
+typedef struct {
+    kr_scalar collapse_mark_time;
+    kr_scalar collapse_time;
+} kr_ads_collapse_metrics;
 
 /* ------------------------------------------------------------------------ */
 /* Legacy entry points (drop-in)                                            */
@@ -154,6 +162,43 @@ kr_error obvh_lbvh_build(const kr_vec3* vertices, const kr_uvec4* faces, u32 fac
                          kr_bvh_node* nodes, u32* parents, u32* primitive_counts,
                          kr_scalar* costs, u32* mortons_sorted,
                          void* workspace, size_t workspace_bytes, void* stream);
+
+/* BVH collapse, replaces kr_cuda_bvh_collapse
+ * (src/ads/common/cuda/transforms.cuh:40-50, body transforms.cu:18-211,
+ * 1061-1125).  Device pointers, 2N-1 entries each except primitives (N):
+ * bottom-up, an internal node with count = counts[l] + counts[r] <= 8 and
+ * ct*area*count < ci*area + costs[l] + costs[r] is collapsed; every
+ * collapsed node whose parent is not collapsed becomes a leaf of
+ * collapsed_nodes (a copy of nodes) holding its subtree's distinct primitive
+ * ids in primitives[primitivesOffset ...]; children of collapsed nodes get
+ * nPrimitives = 0xFFFF.  primitive_counts: leaf entries are read, internal
+ * entries and new leaves written; costs: written for every node.  The
+ * leaves' slices of primitives follow increasing node index (the reference
+ * assigns them in arrival order).  Synchronous; metrics may be NULL.        */
+kr_error kr_cuda_bvh_collapse(const kr_bvh_node* nodes, const u32* parents, kr_bvh_node* collapsed_nodes,
+                              u32* primitive_counts, u32* primitives, kr_scalar* costs,
+                              kr_scalar ci, kr_scalar ct, u32 leaf_count, u32 internal_count,
+                              kr_ads_collapse_metrics* metrics);
+
+/* The same collapse, stream ordered, with a caller-owned workspace.     */
+size_t obvh_collapse_workspace_size(u32 primitive_count);
+kr_error obvh_collapse_async(const kr_bvh_node* nodes, const u32* parents, kr_bvh_node* collapsed_nodes,
+                             u32* primitive_counts, u32* primitives, kr_scalar* costs,
+                             kr_scalar ci, kr_scalar ct, u32 primitive_count,
+                             void* workspace, size_t workspace_bytes, void* stream);
+
+/* Surface-area heuristic of a host tree, replaces kr_bvh_sah
+ * (src/ads/common/transforms.h:38-42, body transforms.cpp:250-303): with
+ * obbs != NULL the OBB surface areas are used (AABB for one-primitive
+ * leaves); nodes marked 0xFFFF are skipped; the quality is appended as
+ * "%f," to outputFile when it is not NULL.  Host code, no GPU.            */
+#ifdef __cplusplus
+kr_scalar kr_bvh_sah(const kr_bvh_node* nodes, kr_i32 root_idx, kr_scalar ci, kr_scalar ct,
+                     const kr_obb3* obbs = nullptr, const char* outputFile = nullptr);
+#else
+kr_scalar kr_bvh_sah(const kr_bvh_node* nodes, kr_i32 root_idx, kr_scalar ci, kr_scalar ct,
+                     const kr_obb3* obbs, const char* outputFile);
+#endif
 
 /* Last error string of the calling host thread (NULL if none). */
 const char* obvh_last_error(void);

@@ -24,6 +24,10 @@ EXPORTS = [
     "obvh_validate_tree_host",
     "obvh_lbvh_workspace_size",
     "obvh_lbvh_build",
+    "kr_cuda_bvh_collapse",
+    "obvh_collapse_workspace_size",
+    "obvh_collapse_async",
+    "kr_bvh_sah",
     "obvh_last_error",
     "obvh_version",
 ]
@@ -77,6 +81,14 @@ def lib():
     L.obvh_lbvh_workspace_size.restype = sz
     L.obvh_lbvh_build.argtypes = [vp, vp, u32, vp, vp, vp, vp, vp, vp, sz, vp]
     L.obvh_lbvh_build.restype = C.c_char_p
+    L.kr_cuda_bvh_collapse.argtypes = [vp, vp, vp, vp, vp, vp, C.c_float, C.c_float, u32, u32, vp]
+    L.kr_cuda_bvh_collapse.restype = C.c_char_p
+    L.obvh_collapse_workspace_size.argtypes = [u32]
+    L.obvh_collapse_workspace_size.restype = sz
+    L.obvh_collapse_async.argtypes = [vp, vp, vp, vp, vp, vp, C.c_float, C.c_float, u32, vp, sz, vp]
+    L.obvh_collapse_async.restype = C.c_char_p
+    L.kr_bvh_sah.argtypes = [vp, C.c_int32, C.c_float, C.c_float, vp, C.c_char_p]
+    L.kr_bvh_sah.restype = C.c_float
     L.obvh_last_error.argtypes = []
     L.obvh_last_error.restype = C.c_char_p
     L.obvh_version.argtypes = []

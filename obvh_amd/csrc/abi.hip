@@ -6,6 +6,7 @@
 #include <mutex>
 #include <vector>
 
+#include "collapse.hpp"
 #include "common.hpp"
 #include "dito.hpp"
 #include "lbvh.hpp"
@@ -273,6 +274,116 @@ kr_error obvh_lbvh_build(const kr_vec3* vertices, const kr_uvec4* faces, u32 fac
     TRY(obvh::lbvh_launch(vertices, faces, face_count, nodes, parents, primitive_counts, costs, mortons_sorted,
                           workspace, workspace_bytes, (hipStream_t)stream));
     return nullptr;
+}
+
+// -------------------------------------------------------- collapse --
+
+size_t obvh_collapse_workspace_size(u32 primitive_count) {
+    obvh::CollapseWs ws;
+    return obvh::collapse_carve(&ws, nullptr, 0, primitive_count);
+}
+
+kr_error obvh_collapse_async(const kr_bvh_node* nodes, const u32* parents, kr_bvh_node* collapsed_nodes,
+                             u32* primitive_counts, u32* primitives, kr_scalar* costs, kr_scalar ci, kr_scalar ct,
+                             u32 primitive_count, void* workspace, size_t workspace_bytes, void* stream) {
+    if (primitive_count == 0) return nullptr;
+    if (primitive_count > 0x7FFFFFFFu) return fail("obvh_collapse: too many primitives");
+    if (!nodes || !parents || !collapsed_nodes || !primitive_counts || !primitives || !costs)
+        return fail("obvh_collapse: NULL array");
+    TRY(obvh::collapse_launch(nodes, parents, collapsed_nodes, primitive_counts, primitives, costs, ci, ct,
+                              primitive_count, workspace, workspace_bytes, (hipStream_t)stream, nullptr));
+    return nullptr;
+}
+
+// Legacy drop-in (transforms.cuh:40-50): synchronous, scratch owned here,
+// metrics (nullable) receive the mark and collapse times in ms.
+kr_error kr_cuda_bvh_collapse(const kr_bvh_node* nodes, const u32* parents, kr_bvh_node* collapsed_nodes,
+                              u32* primitive_counts, u32* primitives, kr_scalar* costs, kr_scalar ci, kr_scalar ct,
+                              u32 leaf_count, u32 internal_count, kr_ads_collapse_metrics* metrics) {
+    if (leaf_count == 0) return nullptr;
+    if (internal_count != leaf_count - 1)
+        return fail("kr_cuda_bvh_collapse: internal_count must be leaf_count - 1");
+    if (leaf_count > 0x7FFFFFFFu) return fail("kr_cuda_bvh_collapse: too many primitives");
+    std::lock_guard<std::mutex> lock(g_mu);
+    int dev = 0;
+    DeviceCache* c = cache_for_current(&dev);
+    if (!c) return fail("kr_cuda_bvh_collapse: no HIP device");
+    obvh::CollapseWs wsd;
+    const size_t need = obvh::collapse_carve(&wsd, nullptr, 0, leaf_count);
+    OBVH_HIP_TRY(grow(&c->ws, &c->ws_bytes, need));
+    hipEvent_t ev[3];
+    for (auto& e : ev) OBVH_HIP_TRY(hipEventCreate(&e));
+    OBVH_HIP_TRY(hipEventRecord(ev[0], nullptr));
+    kr_error r = obvh::collapse_launch(nodes, parents, collapsed_nodes, primitive_counts, primitives, costs, ci, ct,
+                                       leaf_count, c->ws, c->ws_bytes, nullptr, ev[1]);
+    hipEventRecord(ev[2], nullptr);
+    uint32_t err = 0;
+    hipError_t he = hipSuccess;
+    if (!r) {
+        obvh::CollapseWs ws;
+        obvh::collapse_carve(&ws, c->ws, c->ws_bytes, leaf_count);
+        he = hipMemcpy(&err, ws.error, sizeof(err), hipMemcpyDeviceToHost);  // synchronises
+        if (he == hipSuccess && metrics) {
+            float a = 0.0f, b = 0.0f;
+            hipEventElapsedTime(&a, ev[0], ev[1]);
+            hipEventElapsedTime(&b, ev[1], ev[2]);
+            metrics->collapse_mark_time += a;
+            metrics->collapse_time += b;
+        }
+    }
+    for (auto& e : ev) hipEventDestroy(e);
+    TRY(r);
+    if (he != hipSuccess) return obvh_hip_error(he, "kr_cuda_bvh_collapse: hipMemcpy D2H");
+    if (err) return fail("kr_cuda_bvh_collapse: malformed subtree below a collapsed node");
+    return nullptr;
+}
+
+// kr_bvh_sah (src/ads/common/transforms.cpp:250-303), host arrays: a LIFO
+// walk from the root summing ci*area over internal nodes and ct*n*area over
+// leaves (the AABB area for one-primitive leaves even when OBBs are given),
+// skipping nodes marked 0xFFFF, divided by the root area; the quality is
+// appended as "%f," to outputFile when it is not NULL.
+kr_scalar kr_bvh_sah(const kr_bvh_node* nodes, kr_i32 root_idx, kr_scalar ci, kr_scalar ct, const kr_obb3* obbs,
+                     const char* outputFile) {
+    auto aabb_area = [](const kr_bvh_node& n) {
+        const float dx = n.bounds.max.x - n.bounds.min.x, dy = n.bounds.max.y - n.bounds.min.y,
+                    dz = n.bounds.max.z - n.bounds.min.z;
+        return 2.0f * (dx * dy + dx * dz + dy * dz);
+    };
+    auto obb_area = [](const kr_obb3& o) {
+        const float dx = 2.0f * o.ext.x, dy = 2.0f * o.ext.y, dz = 2.0f * o.ext.z;
+        return 2.0f * (dx * dy + dx * dz + dy * dz);
+    };
+    std::vector<int32_t> stack;
+    const kr_bvh_node* root = &nodes[root_idx];
+    float sah = 0.0f;
+    const float root_area = obbs ? obb_area(obbs[root_idx]) : aabb_area(*root);
+    if (root->nPrimitives > 0) sah += ct * root_area * root->nPrimitives;
+    else stack.push_back(root_idx);
+    while (!stack.empty()) {
+        const int32_t idx = stack.back();
+        stack.pop_back();
+        const kr_bvh_node& node = nodes[idx];
+        float area = obbs ? obb_area(obbs[idx]) : aabb_area(node);
+        if (node.nPrimitives > 0) {
+            if (node.nPrimitives == 0xFFFF) continue;
+            if (node.nPrimitives == 1) area = aabb_area(node);
+            sah += ct * node.nPrimitives * area;
+        } else {
+            sah += ci * area;
+            stack.push_back((int32_t)node.u.children.left);
+            stack.push_back((int32_t)node.u.children.right);
+        }
+    }
+    const float quality = sah / root_area;
+    if (outputFile) {
+        FILE* f = fopen(outputFile, "a");
+        if (f) {
+            fprintf(f, "%f,", quality);
+            fclose(f);
+        }
+    }
+    return quality;
 }
 
 }  // extern "C"

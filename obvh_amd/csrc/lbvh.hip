@@ -64,9 +64,7 @@ __global__ void lbvh_init(const kr_vec3* V, uint32_t nf, LbvhWs ws) {
             for (int k = 0; k < 3; ++k) { ws.cbox[k] = 0x7FFFFFFF; ws.cbox[3 + k] = (int32_t)0x80000000; }
         }
     }
-    if (t == 0)
-        for (int k = 0; k < 4; ++k) ws.nlist[k] = 0;
-    for (uint32_t i = t; i < nf; i += gridDim.x * blockDim.x) ws.visit[i] = 0;
+    phase_clear(ws.ph, nf);
 }
 
 __global__ __launch_bounds__(256) void lbvh_centroid_box(const kr_vec3* V, const kr_uvec4* F, uint32_t nf,
@@ -188,83 +186,37 @@ __global__ __launch_bounds__(256) void lbvh_tree(const kr_vec3* V, const kr_uvec
     if (t == 0) parents[0] = kInv;
 }
 
-// Bottom-up bounds / counts / costs (lbvh.cu:399-457) with the same phased
-// arrival protocol as the transform's k-DOP pass: workgroup w of phase k owns
-// leaf positions [w*chunk_k, (w+1)*chunk_k); the second child to arrive at a
-// node continues if the first was completed in an earlier kernel or by its own
-// workgroup, else the node is deferred to the next, coarser phase.  Only
-// workgroup-scope ordering is used inside a kernel.
-__device__ __forceinline__ void lbvh_merge(kr_bvh_node* nodes, uint32_t* counts, float* costs, uint32_t p,
-                                           uint32_t l, uint32_t r) {
-    const float ci = (float)1.2;
-    const kr_bvh_node A = nodes[l], B = nodes[r];
-    Box bn{f3{A.bounds.min.x, A.bounds.min.y, A.bounds.min.z}, f3{A.bounds.max.x, A.bounds.max.y, A.bounds.max.z}};
-    bn = expand3(bn, f3{B.bounds.min.x, B.bounds.min.y, B.bounds.min.z});  // kr_aabb_expand
-    bn = expand3(bn, f3{B.bounds.max.x, B.bounds.max.y, B.bounds.max.z});
-    counts[p] = counts[l] + counts[r];
-    costs[p] = ci * box_area(bn.mn, bn.mx) + costs[r] + costs[l];
-    nodes[p].bounds.min = kr_vec3{bn.mn.x, bn.mn.y, bn.mn.z};
-    nodes[p].bounds.max = kr_vec3{bn.mx.x, bn.mx.y, bn.mx.z};
-}
+// Bottom-up bounds / counts / costs (lbvh.cu:399-457) with the phased
+// arrival protocol of phased.hpp.
+struct LbvhUp {
+    kr_bvh_node* nodes;
+    uint32_t* counts;
+    float* costs;
+    __device__ void leaf(uint32_t cur) {
+        const kr_bvh_node L = nodes[cur];
+        costs[cur] = 1.0f * box_area(f3{L.bounds.min.x, L.bounds.min.y, L.bounds.min.z},
+                                     f3{L.bounds.max.x, L.bounds.max.y, L.bounds.max.z});
+        counts[cur] = 1;
+    }
+    __device__ void merge(uint32_t p, uint32_t l, uint32_t r) {
+        const float ci = (float)1.2;
+        const kr_bvh_node A = nodes[l], B = nodes[r];
+        Box bn{f3{A.bounds.min.x, A.bounds.min.y, A.bounds.min.z},
+               f3{A.bounds.max.x, A.bounds.max.y, A.bounds.max.z}};
+        bn = expand3(bn, f3{B.bounds.min.x, B.bounds.min.y, B.bounds.min.z});  // kr_aabb_expand
+        bn = expand3(bn, f3{B.bounds.max.x, B.bounds.max.y, B.bounds.max.z});
+        counts[p] = counts[l] + counts[r];
+        costs[p] = ci * box_area(bn.mn, bn.mx) + costs[r] + costs[l];
+        nodes[p].bounds.min = kr_vec3{bn.mn.x, bn.mn.y, bn.mn.z};
+        nodes[p].bounds.max = kr_vec3{bn.mx.x, bn.mx.y, bn.mx.z};
+    }
+};
 
 __global__ __launch_bounds__(1024) void lbvh_aabbs(kr_bvh_node* nodes, const uint32_t* parents, uint32_t* counts,
                                                    float* costs, uint32_t nf, LbvhWs ws, uint32_t phase,
                                                    uint32_t chunk) {
-    const uint32_t I = nf - 1;
-    const uint32_t tag = (phase << 24) | (blockIdx.x + 1u);
-    const uint64_t lo = (uint64_t)blockIdx.x * chunk, hi = lo + chunk;
-    const uint32_t nin = phase == 0 ? 0 : min(ws.nlist[phase - 1], I);
-    const uint32_t* in = phase == 0 ? nullptr : ws.list + (size_t)(phase - 1) * (I + 1);
-    uint32_t* out = ws.list + (size_t)phase * (I + 1);
-    const uint32_t nstart = phase == 0 ? chunk : nin;
-    for (uint32_t r0 = threadIdx.x; r0 < nstart; r0 += blockDim.x) {
-        uint32_t cur;
-        uint2 span;
-        if (phase == 0) {
-            const uint64_t pos = lo + r0;
-            if (pos >= nf) break;
-            cur = I + (uint32_t)pos;
-            const kr_bvh_node L = nodes[cur];
-            costs[cur] = 1.0f * box_area(f3{L.bounds.min.x, L.bounds.min.y, L.bounds.min.z},
-                                         f3{L.bounds.max.x, L.bounds.max.y, L.bounds.max.z});
-            counts[cur] = 1;
-            span = make_uint2((uint32_t)pos, (uint32_t)pos);
-        } else {
-            cur = in[r0];
-            if (cur >= I) continue;
-            const uint32_t l = nodes[cur].u.children.left, r = nodes[cur].u.children.right;
-            const uint2 sa = ws.span[l], sb = ws.span[r];
-            span = make_uint2(min(sa.x, sb.x), max(sa.y, sb.y));
-            if (span.x < lo || span.x >= hi) continue;
-            if (span.y >= hi) {
-                const uint32_t slot = atomicAdd(&ws.nlist[phase], 1u);
-                if (slot < I) out[slot] = cur;
-                continue;
-            }
-            lbvh_merge(nodes, counts, costs, cur, l, r);
-        }
-        ws.span[cur] = span;
-        uint32_t p = parents[cur];
-        for (uint32_t step = 0; p != kInv && p < I && step < nf; ++step) {
-            const uint64_t old = __hip_atomic_fetch_add(&ws.visit[p], ((uint64_t)tag << 32) | 1ull,
-                                                        __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-            if ((uint32_t)old == 0) break;
-            const uint32_t t1 = (uint32_t)(old >> 32);
-            if (t1 != tag && (t1 >> 24) >= phase) {
-                const uint32_t slot = atomicAdd(&ws.nlist[phase], 1u);
-                if (slot < I) out[slot] = p;
-                break;
-            }
-            const uint32_t l = nodes[p].u.children.left, r = nodes[p].u.children.right;
-            const uint32_t sib = (l == cur) ? r : l;
-            lbvh_merge(nodes, counts, costs, p, l, r);
-            const uint2 so = ws.span[sib];
-            span = make_uint2(min(span.x, so.x), max(span.y, so.y));
-            ws.span[p] = span;
-            cur = p;
-            p = parents[cur];
-        }
-    }
+    LbvhUp op{nodes, counts, costs};
+    phased_up(nodes, parents, nf, ws.ph, phase, chunk, op);
 }
 
 size_t lbvh_sort_bytes(uint32_t nf) {
@@ -277,10 +229,7 @@ size_t lbvh_sort_bytes(uint32_t nf) {
 size_t lbvh_carve(LbvhWs* ws, void* base, size_t cap, uint32_t nf) {
     Carver c{(char*)base, 0, cap};
     ws->cbox = c.take<int32_t>(8);
-    ws->visit = c.take<uint64_t>(nf + 1);
-    ws->nlist = c.take<uint32_t>(4);
-    ws->list = c.take<uint32_t>(4 * ((size_t)nf + 1));
-    ws->span = c.take<uint2>(2 * (size_t)nf);
+    phase_carve(&ws->ph, c, nf);
     ws->keys_in = c.take<uint32_t>(nf);
     ws->keys_out = c.take<uint32_t>(nf);
     ws->vals_in = c.take<uint32_t>(nf);
@@ -307,15 +256,7 @@ kr_error lbvh_launch(const kr_vec3* V, const kr_uvec4* F, uint32_t nf, kr_bvh_no
     OBVH_HIP_TRY(rocprim::radix_sort_pairs(ws.sort_tmp, sb, ws.keys_in, ws.keys_out, ws.vals_in, ws.vals_out, nf,
                                            0, 30, stream));
     hipLaunchKernelGGL(lbvh_tree, blocks(nf, 256), dim3(256), 0, stream, V, F, nf, ws, nodes, parents, mortons);
-    uint32_t phase = 0;
-    for (uint32_t chunk : {1024u, 32768u, 0u}) {
-        const bool last = (chunk == 0) || chunk >= nf;
-        const uint32_t c = last ? nf : chunk;
-        hipLaunchKernelGGL(lbvh_aabbs, blocks(nf, c), dim3(1024), 0, stream, nodes, parents, counts, costs, nf, ws,
-                           phase, c);
-        ++phase;
-        if (last) break;
-    }
+    phased_launch(lbvh_aabbs, nf, stream, nodes, (const uint32_t*)parents, counts, costs, nf, ws);
     OBVH_HIP_TRY(hipGetLastError());
     return nullptr;
 }

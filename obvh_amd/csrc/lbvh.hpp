@@ -3,15 +3,13 @@
 #include <hip/hip_runtime.h>
 
 #include "obvh/obvh.h"
+#include "phased.hpp"
 
 namespace obvh {
 
 struct LbvhWs {
     int32_t* cbox;       // centroid box as order-preserving keys (min xyz, max xyz)
-    uint64_t* visit;     // arrival words (internal nodes): count + owner tags
-    uint32_t* nlist;     // per phase: number of deferred nodes
-    uint32_t* list;      // per phase: deferred nodes
-    uint2* span;         // per node: first/last leaf position
+    PhaseWs ph;          // bottom-up pass state
     uint32_t* keys_in;   // Morton codes by face
     uint32_t* keys_out;  // sorted codes
     uint32_t* vals_in;   // face ids

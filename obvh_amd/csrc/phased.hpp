@@ -1,0 +1,123 @@
+// phased.hpp -- bottom-up tree passes with the phased arrival protocol.
+//
+// A pass visits every internal node after both of its children, computing
+// the node from its children (bounds, counts, costs, collapse flags ...).
+// Phase k runs workgroups that own leaf positions [w*chunk_k, (w+1)*chunk_k)
+// (chunks 1024, 32768, then one workgroup for everything).  Phase 0 starts
+// one lane per leaf; later phases start from the nodes the previous phase
+// deferred, taken by the workgroup that owns their first leaf.  A lane that
+// completes a child adds (tag << 32) | 1 to the parent's 64-bit arrival word
+// (tag = (phase << 24) | (workgroup + 1)); the first arrival stops, the
+// second continues if the first came from an earlier kernel (visible after
+// the kernel boundary) or from its own workgroup (workgroup-scope acq_rel
+// orders the data), and otherwise defers the parent to the next phase.  Only
+// workgroup-scope ordering is used inside a kernel; no device-scope fence is
+// issued.  The last phase owns every leaf and never defers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+#include "obvh/obvh.h"
+
+namespace obvh {
+
+constexpr int kPhases = 3;  // chunk 1024, chunk 32768, everything
+
+struct PhaseWs {
+    uint64_t* visit;   // per internal node: arrivals (low word) + sum of tags (high word)
+    uint32_t* nlist;   // per phase: number of deferred nodes
+    uint32_t* list;    // per phase: deferred nodes (I + 1 entries each)
+    uint2* span;       // per node: first and last leaf position below it
+};
+
+inline void phase_carve(PhaseWs* ws, Carver& c, uint32_t nf) {
+    ws->visit = c.take<uint64_t>((size_t)nf + 1);
+    ws->nlist = c.take<uint32_t>(4);
+    ws->list = c.take<uint32_t>((size_t)kPhases * ((size_t)nf + 1));
+    ws->span = c.take<uint2>(2 * (size_t)nf);
+}
+
+// Zero the arrival words and the deferred-list counters (grid-stride).
+__device__ __forceinline__ void phase_clear(const PhaseWs& ws, uint32_t nf) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0)
+        for (int k = 0; k < 4; ++k) ws.nlist[k] = 0;
+    for (uint32_t i = t; i < nf; i += gridDim.x * blockDim.x) ws.visit[i] = 0;
+}
+
+// Body of one phase.  Op provides
+//   void leaf(uint32_t node)                       a leaf (phase 0 only)
+//   void merge(uint32_t p, uint32_t l, uint32_t r) an internal node from its children
+template <class Op>
+__device__ __forceinline__ void phased_up(const kr_bvh_node* nodes, const uint32_t* parents, uint32_t nf,
+                                          const PhaseWs& ws, uint32_t phase, uint32_t chunk, Op& op) {
+    constexpr uint32_t kNone = 0xFFFFFFFFu;
+    const uint32_t I = nf - 1;
+    const uint32_t tag = (phase << 24) | (blockIdx.x + 1u);
+    const uint64_t lo = (uint64_t)blockIdx.x * chunk, hi = lo + chunk;
+    const uint32_t nin = phase == 0 ? 0 : min(ws.nlist[phase - 1], I);
+    const uint32_t* in = phase == 0 ? nullptr : ws.list + (size_t)(phase - 1) * (I + 1);
+    uint32_t* out = ws.list + (size_t)phase * (I + 1);
+    const uint32_t nstart = phase == 0 ? chunk : nin;
+    for (uint32_t r0 = threadIdx.x; r0 < nstart; r0 += blockDim.x) {
+        uint32_t cur;
+        uint2 span;
+        if (phase == 0) {
+            const uint64_t pos = lo + r0;
+            if (pos >= nf) break;
+            cur = I + (uint32_t)pos;
+            op.leaf(cur);
+            span = make_uint2((uint32_t)pos, (uint32_t)pos);
+        } else {
+            cur = in[r0];
+            if (cur >= I) continue;
+            const uint32_t l = nodes[cur].u.children.left, r = nodes[cur].u.children.right;
+            const uint2 sa = ws.span[l], sb = ws.span[r];
+            span = make_uint2(min(sa.x, sb.x), max(sa.y, sb.y));
+            if (span.x < lo || span.x >= hi) continue;  // another workgroup owns its first leaf
+            if (span.y >= hi) {                         // wider than this workgroup: next phase
+                const uint32_t slot = atomicAdd(&ws.nlist[phase], 1u);
+                if (slot < I) out[slot] = cur;
+                continue;
+            }
+            op.merge(cur, l, r);
+        }
+        ws.span[cur] = span;
+        uint32_t p = parents[cur];
+        for (uint32_t step = 0; p != kNone && p < I && step < nf; ++step) {
+            const uint32_t l = nodes[p].u.children.left, r = nodes[p].u.children.right;
+            const uint32_t pp = parents[p];
+            const uint64_t old = __hip_atomic_fetch_add(&ws.visit[p], ((uint64_t)tag << 32) | 1ull,
+                                                        __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if ((uint32_t)old == 0) break;  // first arrival: the sibling continues
+            const uint32_t t1 = (uint32_t)(old >> 32);
+            if (t1 != tag && (t1 >> 24) >= phase) {  // sibling done by another workgroup of this phase
+                const uint32_t slot = atomicAdd(&ws.nlist[phase], 1u);
+                if (slot < I) out[slot] = p;
+                break;
+            }
+            const uint32_t sib = (l == cur) ? r : l;
+            op.merge(p, l, r);
+            const uint2 so = ws.span[sib];
+            span = make_uint2(min(span.x, so.x), max(span.y, so.y));
+            ws.span[p] = span;
+            cur = p;
+            p = pp;
+        }
+    }
+}
+
+// Launch the phases of a pass: kernel(args..., phase, chunk) with 1024 threads.
+template <class K, class... A>
+inline void phased_launch(K kernel, uint32_t nf, hipStream_t stream, A... args) {
+    uint32_t phase = 0;
+    for (uint32_t chunk : {1024u, 32768u, 0u}) {
+        const bool last = (chunk == 0) || chunk >= nf;
+        const uint32_t c = last ? nf : chunk;
+        hipLaunchKernelGGL(kernel, dim3((nf + c - 1) / c), dim3(1024), 0, stream, args..., phase, c);
+        ++phase;
+        if (last) break;
+    }
+}
+
+}  // namespace obvh

@@ -164,6 +164,56 @@ def bvh_obb_tree(nodes, parents, V, F, obbs, mats, primitive_count: int, interna
     check(err)
 
 
+# ------------------------------------------------------------ collapse --
+
+class CollapseMetrics(C.Structure):
+    _fields_ = [("collapse_mark_time", C.c_float), ("collapse_time", C.c_float)]
+
+
+def collapse(nodes, parents, counts, ci: float = 1.2, ct: float = 1.0, stream=None, workspace=None):
+    """Stream-ordered BVH collapse of a device tree (nodes uint8 (2N-1)*36 B,
+    parents/counts int32 tensors; counts' leaf entries are read).
+
+    Returns dict(collapsed (uint8 tensor, node layout), counts (int32, the
+    updated copy), costs (float32), primitives (int32, N entries; the slices
+    in use are those of the collapsed tree's leaves)).
+    """
+    torch = _torch()
+    total = int(parents.numel())
+    N = (total + 1) // 2
+    out = torch.empty(total * NODE_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+    cnt = counts.clone()
+    costs = torch.empty(total, dtype=torch.float32, device="cuda")
+    prims = torch.full((N,), -1, dtype=torch.int32, device="cuda")
+    need = lib().obvh_collapse_workspace_size(N)
+    ws = workspace if workspace is not None else _workspace(need)
+    if ws.numel() < need:
+        raise ValueError(f"workspace needs {need} bytes")
+    check(lib().obvh_collapse_async(_ptr(nodes), _ptr(parents), _ptr(out), _ptr(cnt), _ptr(prims), _ptr(costs),
+                                    C.c_float(ci), C.c_float(ct), N, _ptr(ws), ws.numel(), _stream(stream)))
+    return {"collapsed": out, "counts": cnt, "costs": costs, "primitives": prims}
+
+
+def bvh_collapse(nodes, parents, collapsed, counts, primitives, costs, ci: float = 1.2, ct: float = 1.0):
+    """The legacy synchronous kr_cuda_bvh_collapse on device tensors; returns the metrics (ms)."""
+    total = int(parents.numel())
+    N = (total + 1) // 2
+    m = CollapseMetrics(0.0, 0.0)
+    check(lib().kr_cuda_bvh_collapse(_ptr(nodes), _ptr(parents), _ptr(collapsed), _ptr(counts), _ptr(primitives),
+                                     _ptr(costs), C.c_float(ci), C.c_float(ct), N, N - 1, C.byref(m)))
+    return {"collapse_mark_time": m.collapse_mark_time, "collapse_time": m.collapse_time}
+
+
+def bvh_sah(nodes: np.ndarray, root: int = 0, ci: float = 1.2, ct: float = 1.0, obbs=None,
+            output_file: str | None = None) -> float:
+    """kr_bvh_sah on host arrays (NODE_DTYPE nodes, optional (2N-1, 15) float32 OBBs)."""
+    nodes = np.ascontiguousarray(nodes, dtype=NODE_DTYPE)
+    ob = None if obbs is None else np.ascontiguousarray(obbs, dtype=np.float32)
+    return float(lib().kr_bvh_sah(nodes.ctypes.data_as(C.c_void_p), int(root), C.c_float(ci), C.c_float(ct),
+                                  None if ob is None else ob.ctypes.data_as(C.c_void_p),
+                                  output_file.encode() if output_file else None))
+
+
 def validate_tree(nodes: np.ndarray, parents: np.ndarray) -> None:
     """Host-side structural check of a tree (raises ObvhError)."""
     nodes = np.ascontiguousarray(nodes, dtype=NODE_DTYPE)

@@ -42,6 +42,10 @@ def lib():
         L.oracle_lbvh_build.restype = C.c_int
         L.oracle_cpu_obb_tree.argtypes = [vp, vp, vp, vp, C.c_uint32, vp, vp, C.c_uint32, C.c_uint32]
         L.oracle_cpu_obb_tree.restype = C.c_int64
+        L.oracle_collapse.argtypes = [vp, vp, vp, vp, vp, vp, vp, C.c_float, C.c_float, C.c_uint32]
+        L.oracle_collapse.restype = C.c_int64
+        L.oracle_bvh_sah.argtypes = [vp, C.c_int32, C.c_float, C.c_float, vp, C.c_uint32]
+        L.oracle_bvh_sah.restype = C.c_float
         L.oracle_minverse4.argtypes = [vp, vp]
         L.oracle_mmul4.argtypes = [vp, vp, vp]
         _lib = L
@@ -144,6 +148,35 @@ def cpu_obb_tree(nodes, parents, vertices, faces, leaf_begin: int = 0, leaf_end:
     le = N if leaf_end is None else leaf_end
     fitted = lib().oracle_cpu_obb_tree(_p(nodes), _p(parents), _p(V), _p(F), N, _p(obbs), _p(mats), leaf_begin, le)
     return obbs, mats, nodes, int(fitted)
+
+
+def collapse(nodes, parents, counts, ci: float = 1.2, ct: float = 1.0):
+    """kr_cuda_bvh_collapse.  nodes: NODE_DTYPE (2N-1), parents uint32,
+    counts uint32 (leaf entries used).  Returns dict(collapsed, counts, costs,
+    primitives, table, used)."""
+    nodes = np.ascontiguousarray(nodes, dtype=NODE_DTYPE)
+    total = nodes.shape[0]
+    N = (total + 1) // 2
+    par = np.ascontiguousarray(parents, dtype=np.uint32)
+    cnt = np.ascontiguousarray(counts, dtype=np.uint32).copy()
+    costs = np.zeros(total, dtype=np.float32)
+    out = np.zeros(total, dtype=NODE_DTYPE)
+    prims = np.full(N, 0xFFFFFFFF, dtype=np.uint32)
+    table = np.zeros(total, dtype=np.uint8)
+    used = lib().oracle_collapse(_p(nodes), _p(par), _p(cnt), _p(costs), _p(out), _p(prims), _p(table),
+                                 C.c_float(ci), C.c_float(ct), N)
+    if used < 0:
+        raise ValueError("malformed tree")
+    return {"collapsed": out, "counts": cnt, "costs": costs, "primitives": prims[:used], "table": table,
+            "used": int(used)}
+
+
+def bvh_sah(nodes, root: int = 0, ci: float = 1.2, ct: float = 1.0, obbs=None) -> float:
+    """kr_bvh_sah (AABB areas, or OBB areas when obbs (2N-1, 15) is given)."""
+    nodes = np.ascontiguousarray(nodes, dtype=NODE_DTYPE)
+    ob = None if obbs is None else _f32(obbs)
+    return float(lib().oracle_bvh_sah(_p(nodes), root, C.c_float(ci), C.c_float(ct),
+                                      None if ob is None else _p(ob), nodes.shape[0]))
 
 
 def minverse4(m):

@@ -21,6 +21,9 @@
  *                    src/ads/common/cuda/transforms.cu:1127-1250
  *   cpu_transform.c  the reference CPU transform kr_bvh_obb_tree (full
  *                    DiTO per node), src/ads/common/transforms.cpp:349-481
+ *   collapse.c       BVH collapse and SAH, src/ads/common/cuda/
+ *                    transforms.cu:18-211,1061-1125,
+ *                    src/ads/common/transforms.cpp:250-303
  */
 #ifndef OBVH_ORACLE_H
 #define OBVH_ORACLE_H
@@ -82,6 +85,19 @@ int64_t oracle_cpu_obb_tree(onode* nodes, const uint32_t* parents,
                             const float* vertices, const uint32_t* faces,
                             uint32_t nprim, oobb* obbs, omat4* mats,
                             uint32_t leaf_begin, uint32_t leaf_end);
+
+/* kr_cuda_bvh_collapse.  nodes/parents: 2N-1 entries.  counts: leaf entries
+ * read (primitives per leaf), internal entries and collapsed leaves written.
+ * costs (2N-1), collapsed (2N-1, a copy of nodes with the collapse edits),
+ * primitives (N), table (2N-1 flags, nullable).  Returns the number of
+ * primitive slots used, or -1 on a malformed tree. */
+int64_t oracle_collapse(const onode* nodes, const uint32_t* parents, uint32_t* counts, float* costs,
+                        onode* collapsed, uint32_t* primitives, uint8_t* table, float ci, float ct,
+                        uint32_t nprim);
+
+/* kr_bvh_sah over `total` nodes; obbs nullable (AABB areas). */
+float oracle_bvh_sah(const onode* nodes, int32_t root_idx, float ci, float ct, const oobb* obbs,
+                     uint32_t total);
 
 /* 4x4 helpers exposed for tests (vecmath.h:946-1075,1214-1234). */
 void oracle_minverse4(const omat4* m, omat4* out);

@@ -1,0 +1,79 @@
+"""Tree helpers shared by the tests: random topologies with AABBs, subtree
+walks.  Plain numpy, no GPU."""
+import numpy as np
+
+from obvh_amd.layouts import NODE_DTYPE
+
+INV = 0xFFFFFFFF
+
+
+def random_tree(n_leaves: int, seed: int):
+    """Random binary tree over shuffled leaves: internal nodes numbered in
+    breadth-first order (root 0), leaves I..2N-2 holding a random face
+    permutation.  Sibling leaves are rarely adjacent."""
+    rng = np.random.default_rng(seed)
+    I = n_leaves - 1
+    nodes = np.zeros(2 * n_leaves - 1, dtype=NODE_DTYPE)
+    parents = np.zeros(2 * n_leaves - 1, dtype=np.uint32)
+    parents[0] = INV
+    leaf_slots = rng.permutation(n_leaves)  # leaf slot of each face
+    queue = [(0, np.arange(n_leaves))]
+    next_internal = 1
+    head = 0
+    while head < len(queue):
+        idx, faces = queue[head]
+        head += 1
+        cut = int(rng.integers(1, len(faces)))
+        kids = []
+        for part in (faces[:cut], faces[cut:]):
+            if len(part) == 1:
+                kids.append(I + int(leaf_slots[part[0]]))
+            else:
+                kids.append(next_internal)
+                queue.append((next_internal, part))
+                next_internal += 1
+        nodes[idx]["a"], nodes[idx]["b"] = kids
+        for k in kids:
+            parents[k] = idx
+    for f in range(n_leaves):
+        L = nodes[I + leaf_slots[f]]
+        L["a"] = f
+        L["nprim"] = 1
+    return nodes, parents
+
+
+def fill_bounds(nodes, V, F):
+    """Leaf AABBs from their triangles, internal AABBs as unions (in place)."""
+    total = nodes.shape[0]
+    I = (total - 1) // 2
+    leaf = nodes[I:]
+    tri = V[F[leaf["a"], :3].astype(np.int64)]  # (N, 3, 3)
+    leaf["bmin"] = tri.min(axis=1)
+    leaf["bmax"] = tri.max(axis=1)
+    order = bfs_order(nodes)
+    for v in order[::-1]:
+        if v < I:
+            a, b = int(nodes[v]["a"]), int(nodes[v]["b"])
+            nodes[v]["bmin"] = np.minimum(nodes[a]["bmin"], nodes[b]["bmin"])
+            nodes[v]["bmax"] = np.maximum(nodes[a]["bmax"], nodes[b]["bmax"])
+    return nodes
+
+
+def bfs_order(nodes, root: int = 0):
+    total = nodes.shape[0]
+    I = (total - 1) // 2
+    order, head = [root], 0
+    while head < len(order):
+        v = order[head]
+        head += 1
+        if v < I:
+            order.append(int(nodes[v]["a"]))
+            order.append(int(nodes[v]["b"]))
+    return order
+
+
+def leaves_right_first(nodes, v, I):
+    """Leaf nodes below v visited right child first (recursive definition)."""
+    if v >= I:
+        return [v]
+    return leaves_right_first(nodes, int(nodes[v]["b"]), I) + leaves_right_first(nodes, int(nodes[v]["a"]), I)
