@@ -190,10 +190,11 @@ def serving_leg(args) -> dict:
     return out
 
 
-def transform_leg(V, F, label: str, steps: int, warmup: int, with_cpu: bool) -> dict:
-    """LBVH of the mesh built on the GPU, then `steps` timed transforms of the
-    resident tree; optionally the CPU oracle on the same tree (bit-exact
-    comparison and single-thread CPU throughput)."""
+def transform_leg(V, F, label: str, steps: int, warmup: int, with_cpu: bool, builder: str = "lbvh") -> dict:
+    """LBVH (or ATRBVH: treelet 7, 2 passes) of the mesh built on the GPU, then
+    `steps` timed transforms of the resident tree, then the collapse; optionally
+    the CPU oracle on the same tree (bit-exact comparison and single-thread
+    CPU throughput)."""
     import numpy as np
     import torch
 
@@ -202,7 +203,19 @@ def transform_leg(V, F, label: str, steps: int, warmup: int, with_cpu: bool) -> 
     N, NV = F.shape[0], V.shape[0]
     Vd = torch.from_numpy(V).cuda()
     Fd = torch.from_numpy(F.view(np.int32)).cuda()
-    tree = ops.lbvh_build(Vd, Fd)
+    build_ms = None
+    if builder == "atrbvh":
+        ops.atrbvh_build(Vd, Fd, 7, 2)  # warm
+        torch.cuda.synchronize()
+        b0, b1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        b0.record()
+        tree = ops.atrbvh_build(Vd, Fd, 7, 2)
+        b1.record()
+        torch.cuda.synchronize()
+        build_ms = b0.elapsed_time(b1)
+        print(f"[bench] atrbvh build {label}: {build_ms:.3f} ms device", file=sys.stderr, flush=True)
+    else:
+        tree = ops.lbvh_build(Vd, Fd)
     total = 2 * N - 1
     obbs = torch.empty((total, 15), dtype=torch.float32, device="cuda")
     mats = torch.empty((total, 16), dtype=torch.float32, device="cuda")
@@ -230,14 +243,32 @@ def transform_leg(V, F, label: str, steps: int, warmup: int, with_cpu: bool) -> 
     achieved = balg / (dev_ms_mean * 1e-3) / 1e9
     print(f"[bench] transform {label}: {N} triangles, {ms_step:.4f} ms/step wall, {dev_ms_mean:.4f} ms device "
           f"(min {dev_ms[0]:.4f}), {value:.1f} Mnodes/s, {achieved:.1f} GB/s", file=sys.stderr, flush=True)
-    out = {"triangles": N, "vertices": NV, "internal_nodes": internal, "steps": steps,
+    # collapse after the transform (ci = 1.2, ct = 1.0) and the tree quality
+    cws = torch.empty(ops.lib().obvh_collapse_workspace_size(N), dtype=torch.uint8, device="cuda")
+    col = ops.collapse(tree["nodes"], tree["parents"], tree["counts"], workspace=cws)
+    torch.cuda.synchronize()
+    c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    c0.record()
+    for _ in range(5):
+        col = ops.collapse(tree["nodes"], tree["parents"], tree["counts"], workspace=cws)
+    c1.record()
+    torch.cuda.synchronize()
+    collapse_ms = c0.elapsed_time(c1) / 5
+    nodes_h = ops.nodes_to_numpy(tree["nodes"])
+    coll_h = ops.nodes_to_numpy(col["collapsed"])
+    obbs_h = obbs.cpu().numpy()
+    sah = {"aabb": round(ops.bvh_sah(nodes_h), 4), "aabb_collapsed": round(ops.bvh_sah(coll_h), 4),
+           "obb_collapsed": round(ops.bvh_sah(coll_h, obbs=obbs_h), 4)}
+    print(f"[bench] collapse {label}: {collapse_ms:.4f} ms device; SAH {sah}", file=sys.stderr, flush=True)
+    out = {"builder": builder, "build_ms": None if build_ms is None else round(build_ms, 4),
+           "triangles": N, "vertices": NV, "internal_nodes": internal, "steps": steps,
+           "collapse_ms": round(collapse_ms, 5), "sah": sah,
            "ms_per_step": round(ms_step, 5), "device_ms": round(dev_ms_mean, 5),
            "Mnodes_per_s": round(value, 2), "bytes_per_call": balg, "achieved_GBs": round(achieved, 1),
            "roofline_frac": round(achieved / HBM_PEAK_GBS, 4), "cpu_baseline": None, "parity": None}
     if with_cpu:
         import oracle
 
-        nodes_h = ops.nodes_to_numpy(tree["nodes"])
         parents_h = tree["parents"].cpu().numpy().view(np.uint32)
         tc = time.perf_counter()
         eo, em = oracle.obb_tree(nodes_h, parents_h, V, F)
@@ -290,11 +321,11 @@ def main() -> int:
     extra = None
     if not args.no_2m:
         extra = {}
-        for name, gen in (("icosphere_f316", lambda: synth.icosphere(316, seed=2)),
-                          ("hairball_lbvh", lambda: synth.hairball(seed=3))):
+        for name, gen, builder in (("icosphere_f316_lbvh", lambda: synth.icosphere(316, seed=2), "lbvh"),
+                                   ("hairball_2M_atrbvh", lambda: synth.hairball(seed=3), "atrbvh")):
             try:
                 V2, F2 = gen()
-                extra[name] = transform_leg(V2, F2, name, max(5, args.steps // 4), 2, not args.no_cpu)
+                extra[name] = transform_leg(V2, F2, name, max(5, args.steps // 4), 2, not args.no_cpu, builder)
                 del V2, F2
             except Exception as e:
                 print(f"[bench] {name} leg failed: {e!r}", file=sys.stderr, flush=True)

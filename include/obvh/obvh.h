@@ -163,6 +163,19 @@ kr_error obvh_lbvh_build(const kr_vec3* vertices, const kr_uvec4* faces, u32 fac
                          kr_scalar* costs, u32* mortons_sorted,
                          void* workspace, size_t workspace_bytes, void* stream);
 
+/* ATRBVH builder (the atrbvh_gpu plugin's build, src/ads/atrbvh_gpu/
+ * atrbvh.cu:300-430): 63-bit Morton LBVH over the triangle-box centres
+ * normalised by the scene box, then `iterations` passes of the agglomerative
+ * treelet optimiser (treelet_size 3..20 leaves, gamma = treelet_size doubled
+ * per pass, costs ci/ct), then bounds / counts / costs as the plugin's
+ * calculate_aabbs_kernel.  Device pointers; nodes/parents/primitive_counts/
+ * costs have 2*face_count-1 entries.  Stream ordered, caller workspace.   */
+size_t obvh_atrbvh_workspace_size(u32 face_count);
+kr_error obvh_atrbvh_build(const kr_vec3* vertices, const kr_uvec4* faces, u32 face_count,
+                           int treelet_size, int iterations, kr_scalar ci, kr_scalar ct,
+                           kr_bvh_node* nodes, u32* parents, u32* primitive_counts, kr_scalar* costs,
+                           void* workspace, size_t workspace_bytes, void* stream);
+
 /* BVH collapse, replaces kr_cuda_bvh_collapse
  * (src/ads/common/cuda/transforms.cuh:40-50, body transforms.cu:18-211,
  * 1061-1125).  Device pointers, 2N-1 entries each except primitives (N):

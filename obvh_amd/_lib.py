@@ -24,6 +24,8 @@ EXPORTS = [
     "obvh_validate_tree_host",
     "obvh_lbvh_workspace_size",
     "obvh_lbvh_build",
+    "obvh_atrbvh_workspace_size",
+    "obvh_atrbvh_build",
     "kr_cuda_bvh_collapse",
     "obvh_collapse_workspace_size",
     "obvh_collapse_async",
@@ -81,6 +83,11 @@ def lib():
     L.obvh_lbvh_workspace_size.restype = sz
     L.obvh_lbvh_build.argtypes = [vp, vp, u32, vp, vp, vp, vp, vp, vp, sz, vp]
     L.obvh_lbvh_build.restype = C.c_char_p
+    L.obvh_atrbvh_workspace_size.argtypes = [u32]
+    L.obvh_atrbvh_workspace_size.restype = sz
+    L.obvh_atrbvh_build.argtypes = [vp, vp, u32, C.c_int, C.c_int, C.c_float, C.c_float, vp, vp, vp, vp, vp, sz,
+                                    vp]
+    L.obvh_atrbvh_build.restype = C.c_char_p
     L.kr_cuda_bvh_collapse.argtypes = [vp, vp, vp, vp, vp, vp, C.c_float, C.c_float, u32, u32, vp]
     L.kr_cuda_bvh_collapse.restype = C.c_char_p
     L.obvh_collapse_workspace_size.argtypes = [u32]

@@ -6,6 +6,7 @@
 #include <mutex>
 #include <vector>
 
+#include "atrbvh.hpp"
 #include "collapse.hpp"
 #include "common.hpp"
 #include "dito.hpp"
@@ -384,6 +385,25 @@ kr_scalar kr_bvh_sah(const kr_bvh_node* nodes, kr_i32 root_idx, kr_scalar ci, kr
         }
     }
     return quality;
+}
+
+// ---------------------------------------------------------- ATRBVH --
+
+size_t obvh_atrbvh_workspace_size(u32 face_count) {
+    obvh::AtrWs ws;
+    return obvh::atr_carve(&ws, nullptr, 0, face_count);
+}
+
+kr_error obvh_atrbvh_build(const kr_vec3* vertices, const kr_uvec4* faces, u32 face_count, int treelet_size,
+                           int iterations, kr_scalar ci, kr_scalar ct, kr_bvh_node* nodes, u32* parents,
+                           u32* primitive_counts, kr_scalar* costs, void* workspace, size_t workspace_bytes,
+                           void* stream) {
+    if (!vertices || !faces || !nodes || !parents || !primitive_counts || !costs)
+        return fail("obvh_atrbvh_build: NULL array");
+    if (face_count > 0x7FFFFFFFu) return fail("obvh_atrbvh_build: too many faces");
+    TRY(obvh::atr_launch(vertices, faces, face_count, treelet_size, iterations, ci, ct, nodes, parents,
+                         primitive_counts, costs, workspace, workspace_bytes, (hipStream_t)stream));
+    return nullptr;
 }
 
 }  // extern "C"

@@ -128,6 +128,27 @@ def lbvh_build(V, F, stream=None):
     return {"nodes": nodes, "parents": parents, "counts": counts, "costs": costs, "mortons": mortons}
 
 
+def atrbvh_build(V, F, treelet_size: int = 7, iterations: int = 2, ci: float = 1.2, ct: float = 1.0, stream=None,
+                 workspace=None):
+    """Device ATRBVH over (V,3) float32 vertices and (F,4) int32 faces.
+    Returns dict(nodes, parents, counts, costs) like lbvh_build (no mortons)."""
+    torch = _torch()
+    nf = int(F.shape[0])
+    total = 2 * nf - 1
+    nodes = torch.empty(total * NODE_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+    parents = torch.empty(total, dtype=torch.int32, device="cuda")
+    counts = torch.empty(total, dtype=torch.int32, device="cuda")
+    costs = torch.empty(total, dtype=torch.float32, device="cuda")
+    need = lib().obvh_atrbvh_workspace_size(nf)
+    ws = workspace if workspace is not None else _workspace(need)
+    if ws.numel() < need:
+        raise ValueError(f"workspace needs {need} bytes")
+    check(lib().obvh_atrbvh_build(_ptr(V), _ptr(F), nf, int(treelet_size), int(iterations), C.c_float(ci),
+                                  C.c_float(ct), _ptr(nodes), _ptr(parents), _ptr(counts), _ptr(costs), _ptr(ws),
+                                  ws.numel(), _stream(stream)))
+    return {"nodes": nodes, "parents": parents, "counts": counts, "costs": costs}
+
+
 def nodes_to_numpy(nodes) -> np.ndarray:
     return nodes.cpu().numpy().view(NODE_DTYPE)
 

@@ -42,6 +42,9 @@ def lib():
         L.oracle_lbvh_build.restype = C.c_int
         L.oracle_cpu_obb_tree.argtypes = [vp, vp, vp, vp, C.c_uint32, vp, vp, C.c_uint32, C.c_uint32]
         L.oracle_cpu_obb_tree.restype = C.c_int64
+        L.oracle_atrbvh_build.argtypes = [vp, vp, C.c_uint32, C.c_int, C.c_int, C.c_float, C.c_float,
+                                          vp, vp, vp, vp]
+        L.oracle_atrbvh_build.restype = C.c_int
         L.oracle_collapse.argtypes = [vp, vp, vp, vp, vp, vp, vp, C.c_float, C.c_float, C.c_uint32]
         L.oracle_collapse.restype = C.c_int64
         L.oracle_bvh_sah.argtypes = [vp, C.c_int32, C.c_float, C.c_float, vp, C.c_uint32]
@@ -148,6 +151,22 @@ def cpu_obb_tree(nodes, parents, vertices, faces, leaf_begin: int = 0, leaf_end:
     le = N if leaf_end is None else leaf_end
     fitted = lib().oracle_cpu_obb_tree(_p(nodes), _p(parents), _p(V), _p(F), N, _p(obbs), _p(mats), leaf_begin, le)
     return obbs, mats, nodes, int(fitted)
+
+
+def atrbvh_build(vertices, faces, treelet_size: int = 7, iterations: int = 2, ci: float = 1.2, ct: float = 1.0):
+    """ATRBVH.  Returns (nodes, parents, counts, costs) like lbvh_build (no mortons)."""
+    V = _f32(vertices).reshape(-1, 3)
+    F = np.ascontiguousarray(faces, dtype=np.uint32).reshape(-1, 4)
+    n = F.shape[0]
+    nodes = np.zeros(2 * n - 1, dtype=NODE_DTYPE)
+    parents = np.zeros(2 * n - 1, dtype=np.uint32)
+    counts = np.zeros(2 * n - 1, dtype=np.uint32)
+    costs = np.zeros(2 * n - 1, dtype=np.float32)
+    rc = lib().oracle_atrbvh_build(_p(V), _p(F), n, treelet_size, iterations, C.c_float(ci), C.c_float(ct),
+                                   _p(nodes), _p(parents), _p(counts), _p(costs))
+    if rc != 0:
+        raise ValueError("oracle_atrbvh_build failed")
+    return nodes, parents, counts, costs
 
 
 def collapse(nodes, parents, counts, ci: float = 1.2, ct: float = 1.0):

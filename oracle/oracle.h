@@ -21,6 +21,8 @@
  *                    src/ads/common/cuda/transforms.cu:1127-1250
  *   cpu_transform.c  the reference CPU transform kr_bvh_obb_tree (full
  *                    DiTO per node), src/ads/common/transforms.cpp:349-481
+ *   atrbvh.c         ATRBVH builder (64-bit Morton LBVH + agglomerative
+ *                    treelet optimiser), src/ads/atrbvh_gpu/
  *   collapse.c       BVH collapse and SAH, src/ads/common/cuda/
  *                    transforms.cu:18-211,1061-1125,
  *                    src/ads/common/transforms.cpp:250-303
@@ -85,6 +87,12 @@ int64_t oracle_cpu_obb_tree(onode* nodes, const uint32_t* parents,
                             const float* vertices, const uint32_t* faces,
                             uint32_t nprim, oobb* obbs, omat4* mats,
                             uint32_t leaf_begin, uint32_t leaf_end);
+
+/* ATRBVH build (treelet_size 3..20, `iterations` optimiser passes, gamma =
+ * treelet_size doubled per pass).  Outputs as oracle_lbvh_build. */
+int oracle_atrbvh_build(const float* vertices, const uint32_t* faces, uint32_t nfaces, int treelet_size,
+                        int iterations, float ci, float ct, onode* nodes, uint32_t* parents,
+                        uint32_t* counts, float* costs);
 
 /* kr_cuda_bvh_collapse.  nodes/parents: 2N-1 entries.  counts: leaf entries
  * read (primitives per leaf), internal entries and collapsed leaves written.
