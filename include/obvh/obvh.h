@@ -16,6 +16,10 @@
  *   kr_bvh_node                              src/common/ads.h:131-142
  *   kr_error (NULL = success)                src/common/vecmath.h:73,21-24
  *   kr_ads_collapse_metrics                  src/common/ads.h:210-214
+ *   kr_vec2 / kr_mat43 / kr_ivec4            src/common/vecmath.h:135,159,188
+ *   kr_ray / kr_intersection                 src/common/ads.h:64-76
+ *   kr_bvh_node_packed                       src/common/ads.h:144-164
+ *   kr_bvh_transformation_pair / kr_SoA_OBVH src/common/ads.h:166-190
  */
 #ifndef OBVH_OBVH_H
 #define OBVH_OBVH_H
@@ -56,6 +60,40 @@ typedef struct {
     uint8_t axis;
     uint8_t pad_;
 } kr_bvh_node;
+
+typedef struct { float x, y; } kr_vec2;                    /* 8 B */
+typedef struct { float v[12]; } kr_mat43;                  /* 48 B, 4 columns of 3 */
+typedef struct { int32_t x, y, z, w; } kr_ivec4;           /* 16 B */
+
+/* Packed OBVH node (64 B): child boxes, then metadata.  Internal node:
+ * metadata.x / .y = left / right child, each as its bitwise NOT when the
+ * child is a leaf; leaf: x = primitivesOffset, y = x + nPrimitives;
+ * z = axis; w unused.                                                   */
+typedef struct {
+    kr_aabb3 lbounds, rbounds;
+    kr_ivec4 metadata;
+} kr_bvh_node_packed;
+typedef struct { kr_mat4 l, r; } kr_bvh_transformation_pair;  /* 128 B */
+/* Structure-of-arrays OBVH: per internal node the left / right child's
+ * world -> unit-cube matrix (3x4), per node the metadata above.          */
+typedef struct {
+    kr_mat43* left_T;
+    kr_mat43* right_T;
+    kr_ivec4* metadata;
+} kr_SoA_OBVH;
+
+typedef struct {
+    kr_vec3 origin;
+    kr_scalar tmin;
+    kr_vec3 direction;
+    kr_scalar tmax;
+} kr_ray;                                                  /* 32 B */
+typedef struct {
+    kr_vec3 geom_normal;
+    kr_vec2 barys;
+    u32 primitive;   /* 0xFFFFFFFF: no hit */
+    u32 instance;    /* 0; 0xFFFFFFFF if the ray's traversal stack overflowed */
+} kr_intersection;                                         /* 28 B */
 
 /* Collapse timings (ms), accumulated by kr_cuda_bvh_collapse. */
 typedef struct {
@@ -213,6 +251,74 @@ kr_scalar kr_bvh_sah(const kr_bvh_node* nodes, kr_i32 root_idx, kr_scalar ci, kr
                      const kr_obb3* obbs, const char* outputFile);
 #endif
 
+/* ------------------------------------------------------------------------ */
+/* OBVH packing and closest-hit traversal                                   */
+/* ------------------------------------------------------------------------ */
+
+/* Packs a (plain or collapsed) tree and its transforms into AoS form.
+ * Replaces kr_cuda_obvh_persistent (src/ads/common/cuda/transforms.cuh:
+ * 32-38, body transforms.cu:1298-1364).  Device pointers; packed_bvh has
+ * node_count entries, packed_transforms one per internal node (indexed by
+ * node).  Nodes marked 0xFFFF and the slots they would fill are left
+ * untouched, as in the reference.  Synchronous.                          */
+kr_error kr_cuda_obvh_persistent(const kr_bvh_node* nodes, const kr_mat4* transforms,
+                                 kr_bvh_node_packed* packed_nodes,
+                                 kr_bvh_transformation_pair* packed_transforms, u32 node_count);
+
+/* Builds the structure-of-arrays OBVH from the packed form.  Replaces
+ * kr_cuda_soa_obvh (src/ads/common/cuda/transforms.cuh:72-77, body
+ * transforms.cu:1366-1449), whose third parameter is a kr_SoA_OBVH*&: the
+ * same argument passing as the pointer-to-pointer here.  Allocates the
+ * arrays and a device copy of the kr_SoA_OBVH struct and stores that
+ * device pointer in *soa_bvh.  The conversion runs on the device (the
+ * reference copies to the host and back).  Synchronous.                  */
+kr_error kr_cuda_soa_obvh(const kr_bvh_node_packed* packed_bvh,
+                          const kr_bvh_transformation_pair* packed_transforms,
+                          kr_SoA_OBVH** soa_bvh, u32 internal_count, u32 node_count);
+
+/* Closest-hit traversal of ray_count rays.  Replaces
+ * kr_cuda_soa_obvh_intersect (src/ads/common/cuda/intersectors.cuh:81-91,
+ * body intersectors.cu:801-972,1452-1472).  bvh is the device struct made
+ * by kr_cuda_soa_obvh; primitives is read for leaves of more than one
+ * primitive (the collapsed tree's slices).  block_size, grid_size and
+ * warp_counter are accepted and ignored (the reference ignores the first
+ * two; rays are handed out from the library's own counter).  Stream
+ * ordered on the default stream; returns after the launch.  A ray whose
+ * traversal stack would need a 64th entry stops with its hit so far and
+ * instance = 0xFFFFFFFF (the reference writes past its stack there).     */
+kr_error kr_cuda_soa_obvh_intersect(const kr_SoA_OBVH* bvh, const kr_vec3* vertices,
+                                    const kr_uvec4* faces, const u32* primitives, const kr_ray* rays,
+                                    kr_intersection* isects, u32 ray_count, u32 block_size,
+                                    u32 grid_size, u32* warp_counter);
+
+/* New: frees what kr_cuda_soa_obvh allocated (the reference never does). */
+kr_error obvh_soa_obvh_free(kr_SoA_OBVH* soa_bvh);
+
+/* New: one-pass device packing from the tree and its transforms straight
+ * into caller-owned SoA arrays (left_T / right_T: internal_count entries,
+ * 16-byte aligned; metadata: node_count).  Equals kr_cuda_obvh_persistent
+ * followed by kr_cuda_soa_obvh, except that slots the reference leaves
+ * unwritten (0xFFFF nodes, metadata.w, matrices of non-internal slots)
+ * are zero.  Stream ordered.                                             */
+kr_error obvh_soa_pack_async(const kr_bvh_node* nodes, const kr_mat4* transforms, u32 node_count,
+                             u32 internal_count, kr_mat43* left_T, kr_mat43* right_T,
+                             kr_ivec4* metadata, void* stream);
+
+/* New: the traversal with the SoA arrays passed directly, their sizes
+ * (left_T / right_T: internal_count, metadata: node_count, faces and
+ * primitives: face_count, vertices: vertex_count) and a caller workspace
+ * (ray counter and overflow stack slots); stream ordered.  Every index the
+ * walk meets is checked against those sizes: a ray that meets one outside
+ * them (a malformed tree) stops with instance = 0xFFFFFFFF, as does a ray
+ * that overflows the 63-entry stack.                                     */
+size_t obvh_soa_intersect_workspace_size(u32 ray_count);
+kr_error obvh_soa_intersect_async(const kr_mat43* left_T, const kr_mat43* right_T,
+                                  const kr_ivec4* metadata, u32 internal_count, u32 node_count,
+                                  const kr_vec3* vertices, u32 vertex_count,
+                                  const kr_uvec4* faces, const u32* primitives, u32 face_count,
+                                  const kr_ray* rays, kr_intersection* isects, u32 ray_count,
+                                  void* workspace, size_t workspace_bytes, void* stream);
+
 /* Last error string of the calling host thread (NULL if none). */
 const char* obvh_last_error(void);
 
@@ -228,6 +334,11 @@ static_assert(sizeof(kr_obb3) == 60, "kr_obb3 must be 60 bytes");
 static_assert(sizeof(kr_mat4) == 64, "kr_mat4 must be 64 bytes");
 static_assert(sizeof(kr_bvh_node) == 36, "kr_bvh_node must be 36 bytes");
 static_assert(sizeof(obvh_dito_trace) == 96, "obvh_dito_trace must be 96 bytes");
+static_assert(sizeof(kr_mat43) == 48, "kr_mat43 must be 48 bytes");
+static_assert(sizeof(kr_bvh_node_packed) == 64, "kr_bvh_node_packed must be 64 bytes");
+static_assert(sizeof(kr_bvh_transformation_pair) == 128, "kr_bvh_transformation_pair must be 128 bytes");
+static_assert(sizeof(kr_ray) == 32, "kr_ray must be 32 bytes");
+static_assert(sizeof(kr_intersection) == 28, "kr_intersection must be 28 bytes");
 #endif
 
 #endif /* OBVH_OBVH_H */

@@ -30,6 +30,13 @@ EXPORTS = [
     "obvh_collapse_workspace_size",
     "obvh_collapse_async",
     "kr_bvh_sah",
+    "kr_cuda_obvh_persistent",
+    "kr_cuda_soa_obvh",
+    "kr_cuda_soa_obvh_intersect",
+    "obvh_soa_obvh_free",
+    "obvh_soa_pack_async",
+    "obvh_soa_intersect_workspace_size",
+    "obvh_soa_intersect_async",
     "obvh_last_error",
     "obvh_version",
 ]
@@ -96,6 +103,20 @@ def lib():
     L.obvh_collapse_async.restype = C.c_char_p
     L.kr_bvh_sah.argtypes = [vp, C.c_int32, C.c_float, C.c_float, vp, C.c_char_p]
     L.kr_bvh_sah.restype = C.c_float
+    L.kr_cuda_obvh_persistent.argtypes = [vp, vp, vp, vp, u32]
+    L.kr_cuda_obvh_persistent.restype = C.c_char_p
+    L.kr_cuda_soa_obvh.argtypes = [vp, vp, C.POINTER(C.c_void_p), u32, u32]
+    L.kr_cuda_soa_obvh.restype = C.c_char_p
+    L.kr_cuda_soa_obvh_intersect.argtypes = [vp, vp, vp, vp, vp, vp, u32, u32, u32, vp]
+    L.kr_cuda_soa_obvh_intersect.restype = C.c_char_p
+    L.obvh_soa_obvh_free.argtypes = [vp]
+    L.obvh_soa_obvh_free.restype = C.c_char_p
+    L.obvh_soa_pack_async.argtypes = [vp, vp, u32, u32, vp, vp, vp, vp]
+    L.obvh_soa_pack_async.restype = C.c_char_p
+    L.obvh_soa_intersect_workspace_size.argtypes = [u32]
+    L.obvh_soa_intersect_workspace_size.restype = sz
+    L.obvh_soa_intersect_async.argtypes = [vp, vp, vp, u32, u32, vp, u32, vp, vp, u32, vp, vp, u32, vp, sz, vp]
+    L.obvh_soa_intersect_async.restype = C.c_char_p
     L.obvh_last_error.argtypes = []
     L.obvh_last_error.restype = C.c_char_p
     L.obvh_version.argtypes = []

@@ -4,6 +4,7 @@
 #include <string.h>
 
 #include <mutex>
+#include <unordered_map>
 #include <vector>
 
 #include "atrbvh.hpp"
@@ -12,6 +13,7 @@
 #include "dito.hpp"
 #include "lbvh.hpp"
 #include "transform.hpp"
+#include "traverse.hpp"
 
 namespace {
 
@@ -24,9 +26,20 @@ struct DeviceCache {
     void* ws = nullptr;
     size_t ws_bytes = 0;
     void* out = nullptr;  // kr_obb3 + error word
+    void* trav = nullptr;  // traversal workspace of kr_cuda_soa_obvh_intersect
+    size_t trav_bytes = 0;
 };
 std::mutex g_mu;
 std::vector<DeviceCache> g_cache;
+
+// Device kr_SoA_OBVH structs made by kr_cuda_soa_obvh: a host copy of the
+// struct and the array sizes, so that the traversal needs no read-back and
+// can check indices.
+struct SoaInfo {
+    kr_SoA_OBVH host;
+    uint32_t internal_count, node_count;
+};
+std::unordered_map<const kr_SoA_OBVH*, SoaInfo> g_soa;
 
 DeviceCache* cache_for_current(int* dev) {
     if (hipGetDevice(dev) != hipSuccess) return nullptr;
@@ -403,6 +416,146 @@ kr_error obvh_atrbvh_build(const kr_vec3* vertices, const kr_uvec4* faces, u32 f
     if (face_count > 0x7FFFFFFFu) return fail("obvh_atrbvh_build: too many faces");
     TRY(obvh::atr_launch(vertices, faces, face_count, treelet_size, iterations, ci, ct, nodes, parents,
                          primitive_counts, costs, workspace, workspace_bytes, (hipStream_t)stream));
+    return nullptr;
+}
+
+// ------------------------------------------------------- OBVH --
+
+// Legacy drop-in (transforms.cuh:32-38): AoS packing, synchronous.
+kr_error kr_cuda_obvh_persistent(const kr_bvh_node* nodes, const kr_mat4* transforms, kr_bvh_node_packed* packed_nodes,
+                                 kr_bvh_transformation_pair* packed_transforms, u32 node_count) {
+    if (node_count == 0) return nullptr;
+    if (!nodes || !transforms || !packed_nodes || !packed_transforms)
+        return fail("kr_cuda_obvh_persistent: NULL array");
+    if (node_count > 0x7FFFFFFEu) return fail("kr_cuda_obvh_persistent: too many nodes");
+    TRY(obvh::aos_pack_launch(nodes, transforms, packed_nodes, packed_transforms, node_count, nullptr));
+    OBVH_HIP_TRY(hipStreamSynchronize(nullptr));
+    return nullptr;
+}
+
+// Legacy drop-in (transforms.cuh:72-77): allocates the SoA arrays and a
+// device copy of the struct, converts on the device, synchronous.
+kr_error kr_cuda_soa_obvh(const kr_bvh_node_packed* packed_bvh, const kr_bvh_transformation_pair* packed_transforms,
+                          kr_SoA_OBVH** soa_bvh, u32 internal_count, u32 node_count) {
+    if (!soa_bvh) return fail("kr_cuda_soa_obvh: soa_bvh is NULL");
+    if (!packed_bvh || !packed_transforms) return fail("kr_cuda_soa_obvh: NULL array");
+    if (node_count > 0x7FFFFFFEu || internal_count > node_count)
+        return fail("kr_cuda_soa_obvh: bad sizes");
+    kr_SoA_OBVH h{};
+    kr_SoA_OBVH* d = nullptr;
+    auto cleanup = [&]() {
+        hipFree(h.left_T);
+        hipFree(h.right_T);
+        hipFree(h.metadata);
+        hipFree(d);
+    };
+    hipError_t e = hipMalloc((void**)&h.left_T, sizeof(kr_mat43) * (internal_count ? internal_count : 1));
+    if (e == hipSuccess) e = hipMalloc((void**)&h.right_T, sizeof(kr_mat43) * (internal_count ? internal_count : 1));
+    if (e == hipSuccess) e = hipMalloc((void**)&h.metadata, sizeof(kr_ivec4) * (node_count ? node_count : 1));
+    if (e == hipSuccess) e = hipMalloc((void**)&d, sizeof(kr_SoA_OBVH));
+    if (e != hipSuccess) {
+        cleanup();
+        return obvh_hip_error(e, "kr_cuda_soa_obvh: hipMalloc");
+    }
+    kr_error r = obvh::aos_to_soa_launch(packed_bvh, packed_transforms, internal_count, node_count, h.left_T,
+                                         h.right_T, h.metadata, nullptr);
+    if (!r) {
+        e = hipMemcpy(d, &h, sizeof(h), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+        if (e != hipSuccess) r = obvh_hip_error(e, "kr_cuda_soa_obvh");
+    }
+    if (r) {
+        cleanup();
+        TRY(r);
+    }
+    {
+        std::lock_guard<std::mutex> lock(g_mu);
+        g_soa[d] = SoaInfo{h, internal_count, node_count};
+    }
+    *soa_bvh = d;
+    return nullptr;
+}
+
+kr_error obvh_soa_obvh_free(kr_SoA_OBVH* soa_bvh) {
+    if (!soa_bvh) return nullptr;
+    kr_SoA_OBVH h{};
+    {
+        std::lock_guard<std::mutex> lock(g_mu);
+        auto it = g_soa.find(soa_bvh);
+        if (it == g_soa.end()) return fail("obvh_soa_obvh_free: not made by kr_cuda_soa_obvh");
+        h = it->second.host;
+        g_soa.erase(it);
+    }
+    hipError_t e = hipFree(h.left_T);
+    if (e == hipSuccess) e = hipFree(h.right_T);
+    if (e == hipSuccess) e = hipFree(h.metadata);
+    if (e == hipSuccess) e = hipFree(soa_bvh);
+    if (e != hipSuccess) return obvh_hip_error(e, "obvh_soa_obvh_free");
+    return nullptr;
+}
+
+// Legacy drop-in (intersectors.cuh:81-91).  For a struct made by
+// kr_cuda_soa_obvh the array sizes are known and every index is checked;
+// for any other device struct it is read back and only the face/vertex
+// checks are off (sizes unknown), with a visit cap against cycles.
+kr_error kr_cuda_soa_obvh_intersect(const kr_SoA_OBVH* bvh, const kr_vec3* vertices, const kr_uvec4* faces,
+                                    const u32* primitives, const kr_ray* rays, kr_intersection* isects, u32 ray_count,
+                                    u32 block_size, u32 grid_size, u32* warp_counter) {
+    (void)block_size;
+    (void)grid_size;
+    (void)warp_counter;
+    if (ray_count == 0) return nullptr;
+    if (!bvh || !vertices || !faces || !rays || !isects) return fail("kr_cuda_soa_obvh_intersect: NULL argument");
+    std::lock_guard<std::mutex> lock(g_mu);
+    obvh::TravCounts counts{0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 1u << 24};
+    kr_SoA_OBVH h{};
+    auto it = g_soa.find(bvh);
+    if (it != g_soa.end()) {
+        h = it->second.host;
+        counts.internal_count = it->second.internal_count;
+        counts.node_count = it->second.node_count;
+        counts.max_steps = it->second.node_count;
+    } else {
+        OBVH_HIP_TRY(hipMemcpy(&h, bvh, sizeof(h), hipMemcpyDeviceToHost));
+    }
+    int dev = 0;
+    DeviceCache* c = cache_for_current(&dev);
+    if (!c) return fail("kr_cuda_soa_obvh_intersect: no HIP device");
+    obvh::TravWs ws;
+    OBVH_HIP_TRY(grow(&c->trav, &c->trav_bytes, obvh::trav_carve(&ws, nullptr, 0, ray_count)));
+    TRY(obvh::intersect_launch(h.left_T, h.right_T, h.metadata, vertices, faces, primitives, rays, isects, ray_count,
+                               counts, c->trav, c->trav_bytes, nullptr));
+    return nullptr;
+}
+
+kr_error obvh_soa_pack_async(const kr_bvh_node* nodes, const kr_mat4* transforms, u32 node_count, u32 internal_count,
+                             kr_mat43* left_T, kr_mat43* right_T, kr_ivec4* metadata, void* stream) {
+    if (node_count == 0) return nullptr;
+    if (!nodes || !transforms || !metadata || (internal_count && (!left_T || !right_T)))
+        return fail("obvh_soa_pack: NULL array");
+    if (node_count > 0x7FFFFFFEu || internal_count > node_count) return fail("obvh_soa_pack: bad sizes");
+    TRY(obvh::soa_pack_launch(nodes, transforms, node_count, internal_count, left_T, right_T, metadata,
+                              (hipStream_t)stream));
+    return nullptr;
+}
+
+size_t obvh_soa_intersect_workspace_size(u32 ray_count) {
+    obvh::TravWs ws;
+    return obvh::trav_carve(&ws, nullptr, 0, ray_count);
+}
+
+kr_error obvh_soa_intersect_async(const kr_mat43* left_T, const kr_mat43* right_T, const kr_ivec4* metadata,
+                                  u32 internal_count, u32 node_count, const kr_vec3* vertices, u32 vertex_count,
+                                  const kr_uvec4* faces, const u32* primitives, u32 face_count, const kr_ray* rays,
+                                  kr_intersection* isects, u32 ray_count, void* workspace, size_t workspace_bytes,
+                                  void* stream) {
+    if (ray_count == 0) return nullptr;
+    if (!left_T || !right_T || !metadata || !vertices || !faces || !rays || !isects)
+        return fail("obvh_soa_intersect: NULL argument");
+    if (internal_count == 0 || node_count == 0) return fail("obvh_soa_intersect: the tree needs an internal root");
+    obvh::TravCounts counts{internal_count, node_count, face_count, vertex_count, node_count};
+    TRY(obvh::intersect_launch(left_T, right_T, metadata, vertices, faces, primitives, rays, isects, ray_count, counts,
+                               workspace, workspace_bytes, (hipStream_t)stream));
     return nullptr;
 }
 

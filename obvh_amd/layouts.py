@@ -2,8 +2,10 @@
 
 Layouts follow the reference types (paths relative to the reference repo):
 ``kr_vec3`` 12 B, ``kr_uvec4`` 16 B, ``kr_obb3`` 60 B (mid, v0, v1, v2, ext),
-``kr_mat4`` 64 B column major (src/common/vecmath.h:136,156-158,172) and
-``kr_bvh_node`` 36 B (src/common/ads.h:131-142).
+``kr_mat4`` 64 B column major (src/common/vecmath.h:136,156-158,172),
+``kr_bvh_node`` 36 B (src/common/ads.h:131-142), ``kr_ray`` 32 B and
+``kr_intersection`` 28 B (src/common/ads.h:64-76), ``kr_mat43`` 48 B
+(vecmath.h:159) and ``kr_ivec4`` 16 B (vecmath.h:188).
 """
 from __future__ import annotations
 
@@ -41,6 +43,15 @@ TRACE_DTYPE = np.dtype(
     ]
 )
 assert TRACE_DTYPE.itemsize == 96
+
+RAY_DTYPE = np.dtype([("origin", "<f4", (3,)), ("tmin", "<f4"), ("direction", "<f4", (3,)), ("tmax", "<f4")])
+assert RAY_DTYPE.itemsize == 32
+
+ISECT_DTYPE = np.dtype([("normal", "<f4", (3,)), ("barys", "<f4", (2,)), ("primitive", "<u4"), ("instance", "<u4")])
+assert ISECT_DTYPE.itemsize == 28
+
+MISS = 0xFFFFFFFF          # kr_intersection.primitive of a ray that hit nothing
+STACK_OVERFLOW = 0xFFFFFFFF  # kr_intersection.instance of a ray whose traversal stack overflowed
 
 OBB_FLOATS = 15  # mid, v0, v1, v2, ext
 MAT_FLOATS = 16

@@ -9,6 +9,12 @@ Mirrors the reference operator interface for the OBB path:
 * ``dito(x, y, z)``, ``dito_batch`` device-resident, stream-ordered DiTO.
 * ``lbvh_build(V, F)``              device LBVH (lbvh.cu:1547-1692).
 * ``obb_tree(...)``                 stream-ordered transform with workspace.
+* ``obvh_persistent`` / ``soa_obvh`` / ``soa_obvh_intersect``
+                                    the packing and traversal drop-ins
+                                    (transforms.cuh:32-38,72-77,
+                                    intersectors.cuh:81-91).
+* ``soa_pack``, ``soa_intersect``   one-pass packing and the stream-ordered
+                                    traversal with workspace.
 
 Device memory and streams come from torch (plumbing only); every compute step
 runs in libobvh.so.  Errors raise ObvhError; there is no CPU fallback.
@@ -20,7 +26,7 @@ import ctypes as C
 import numpy as np
 
 from ._lib import ObvhError, check, lib
-from .layouts import NODE_DTYPE, TRACE_DTYPE
+from .layouts import ISECT_DTYPE, NODE_DTYPE, RAY_DTYPE, TRACE_DTYPE
 
 
 def _torch():
@@ -242,3 +248,77 @@ def validate_tree(nodes: np.ndarray, parents: np.ndarray) -> None:
     N = (nodes.shape[0] + 1) // 2
     check(lib().obvh_validate_tree_host(nodes.ctypes.data_as(C.c_void_p), parents.ctypes.data_as(C.c_void_p),
                                         N, N - 1))
+
+
+# ---------------------------------------------------------------- OBVH --
+
+def soa_pack(nodes, mats, stream=None):
+    """One-pass OBVH packing of a device tree (uint8 node bytes, plain or
+    collapsed) and its transforms ((2N-1, 16) float32).  Returns dict(left_T
+    (N-1, 12), right_T (N-1, 12) float32, meta (2N-1, 4) int32)."""
+    torch = _torch()
+    total = nodes.numel() // NODE_DTYPE.itemsize
+    internal = (total - 1) // 2
+    L = torch.empty((max(internal, 1), 12), dtype=torch.float32, device="cuda")
+    R = torch.empty((max(internal, 1), 12), dtype=torch.float32, device="cuda")
+    meta = torch.empty((total, 4), dtype=torch.int32, device="cuda")
+    check(lib().obvh_soa_pack_async(_ptr(nodes), _ptr(mats), total, internal, _ptr(L), _ptr(R), _ptr(meta),
+                                    _stream(stream)))
+    return {"left_T": L[:internal], "right_T": R[:internal], "meta": meta}
+
+
+def soa_intersect(packed, V, F, rays, primitives=None, stream=None, workspace=None, out=None):
+    """Closest hits of device rays (uint8 tensor of kr_ray records, or a
+    RAY_DTYPE numpy array) against a packed OBVH.  Returns a uint8 tensor of
+    kr_intersection records (view the host copy with ISECT_DTYPE)."""
+    torch = _torch()
+    if isinstance(rays, np.ndarray):
+        rays = torch.from_numpy(np.ascontiguousarray(rays, dtype=RAY_DTYPE).view(np.uint8)).cuda()
+    n = rays.numel() // RAY_DTYPE.itemsize
+    if out is None:
+        out = torch.empty(n * ISECT_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+    need = lib().obvh_soa_intersect_workspace_size(n)
+    ws = workspace if workspace is not None else _workspace(need)
+    if ws.numel() < need:
+        raise ValueError(f"workspace needs {need} bytes")
+    nf = F.numel() // 4
+    check(lib().obvh_soa_intersect_async(_ptr(packed["left_T"]), _ptr(packed["right_T"]), _ptr(packed["meta"]),
+                                         packed["left_T"].shape[0], packed["meta"].shape[0], _ptr(V),
+                                         V.numel() // 3, _ptr(F), _ptr(primitives), nf, _ptr(rays), _ptr(out), n,
+                                         _ptr(ws), ws.numel(), _stream(stream)))
+    return out
+
+
+def obvh_persistent(nodes, mats):
+    """kr_cuda_obvh_persistent (legacy, synchronous): AoS packed nodes (uint8,
+    64 B each) and transformation pairs (uint8, 128 B per internal node).
+    Slots the reference leaves untouched are zero here (fresh buffers)."""
+    torch = _torch()
+    total = nodes.numel() // NODE_DTYPE.itemsize
+    internal = (total - 1) // 2
+    packed = torch.zeros(total * 64, dtype=torch.uint8, device="cuda")
+    pairs = torch.zeros(max(internal, 1) * 128, dtype=torch.uint8, device="cuda")
+    check(lib().kr_cuda_obvh_persistent(_ptr(nodes), _ptr(mats), _ptr(packed), _ptr(pairs), total))
+    return packed, pairs
+
+
+def soa_obvh(packed, pairs, internal_count: int, node_count: int) -> int:
+    """kr_cuda_soa_obvh (legacy): returns the device kr_SoA_OBVH pointer."""
+    h = C.c_void_p(0)
+    check(lib().kr_cuda_soa_obvh(_ptr(packed), _ptr(pairs), C.byref(h), internal_count, node_count))
+    return int(h.value)
+
+
+def soa_obvh_intersect(handle: int, V, F, primitives, rays, out) -> None:
+    """kr_cuda_soa_obvh_intersect (legacy, default stream)."""
+    n = rays.numel() // RAY_DTYPE.itemsize
+    check(lib().kr_cuda_soa_obvh_intersect(C.c_void_p(handle), _ptr(V), _ptr(F), _ptr(primitives), _ptr(rays),
+                                           _ptr(out), n, 128, 0, None))
+
+
+def soa_obvh_free(handle: int) -> None:
+    check(lib().obvh_soa_obvh_free(C.c_void_p(handle)))
+
+
+def isects_to_numpy(isects) -> np.ndarray:
+    return isects.cpu().numpy().view(ISECT_DTYPE)

@@ -11,6 +11,10 @@ generated here, deterministically from a seed (numpy PCG64):
 * ``hairball``          thin open tubes (20 radial segments, 40 triangles and
                         40 vertices each) inside the unit cube.
 * ``batch_sets``        many Gaussian sets with log-uniform sizes.
+* ``camera_rays``       pinhole camera on a ring around the scene box,
+                        one ray per pixel, row-major (coherent queries).
+* ``random_rays``       origins on a sphere around the box aimed at uniform
+                        points inside it (incoherent queries).
 
 Everything is computed in float64 and rounded once to float32.
 """
@@ -202,3 +206,57 @@ def batch_sets(sizes, seed: int = 5):
         p = rng.standard_normal((int(n), 3)) * std
         out[int(off[s]):int(off[s + 1])] = (p @ R[s].T).astype(np.float32)
     return out, off
+
+
+RAY_TMAX = 1.0e30
+
+
+def _rays(origins: np.ndarray, directions: np.ndarray) -> np.ndarray:
+    from .layouts import RAY_DTYPE
+
+    r = np.zeros(origins.shape[0], dtype=RAY_DTYPE)
+    r["origin"] = origins.astype(np.float32)
+    r["direction"] = directions.astype(np.float32)
+    r["tmin"] = 0.0
+    r["tmax"] = RAY_TMAX
+    return r
+
+
+def camera_rays(vertices: np.ndarray, width: int = 1024, height: int = 1024, angle: float = 0.6,
+                fov_deg: float = 40.0, distance: float = 1.8) -> np.ndarray:
+    """Pinhole camera at `distance` box radii from the box centre, on a ring
+    at azimuth `angle` (radians) and 20 degrees elevation, looking at the
+    centre; unit directions through pixel centres, row-major."""
+    v = np.asarray(vertices, dtype=np.float64)
+    lo, hi = v.min(0), v.max(0)
+    c = 0.5 * (lo + hi)
+    rad = 0.5 * float(np.linalg.norm(hi - lo))
+    el = math.radians(20.0)
+    eye = c + distance * rad * np.array([math.cos(el) * math.cos(angle), math.sin(el), math.cos(el) * math.sin(angle)])
+    fwd = (c - eye) / np.linalg.norm(c - eye)
+    right = np.cross(fwd, [0.0, 1.0, 0.0])
+    right /= np.linalg.norm(right)
+    up = np.cross(right, fwd)
+    th = math.tan(math.radians(fov_deg) / 2)
+    ys, xs = np.meshgrid(np.arange(height), np.arange(width), indexing="ij")
+    px = ((xs + 0.5) / width * 2 - 1) * th * (width / height)
+    py = (1 - (ys + 0.5) / height * 2) * th
+    d = fwd[None, :] + px.reshape(-1, 1) * right[None, :] + py.reshape(-1, 1) * up[None, :]
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return _rays(np.broadcast_to(eye, d.shape), d)
+
+
+def random_rays(vertices: np.ndarray, n: int, seed: int = 11) -> np.ndarray:
+    """Origins uniform on the sphere of 1.5 box radii around the box centre,
+    unit directions towards points uniform in the box."""
+    v = np.asarray(vertices, dtype=np.float64)
+    lo, hi = v.min(0), v.max(0)
+    c = 0.5 * (lo + hi)
+    rad = 0.5 * float(np.linalg.norm(hi - lo))
+    rng = _rng(seed)
+    u = rng.standard_normal((n, 3))
+    o = c + 1.5 * rad * u / np.linalg.norm(u, axis=1, keepdims=True)
+    t = lo + rng.uniform(size=(n, 3)) * (hi - lo)
+    d = t - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return _rays(o, d)

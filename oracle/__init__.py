@@ -12,7 +12,7 @@ import subprocess
 
 import numpy as np
 
-from obvh_amd.layouts import NODE_DTYPE, TRACE_DTYPE
+from obvh_amd.layouts import ISECT_DTYPE, NODE_DTYPE, RAY_DTYPE, TRACE_DTYPE
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
@@ -49,6 +49,12 @@ def lib():
         L.oracle_collapse.restype = C.c_int64
         L.oracle_bvh_sah.argtypes = [vp, C.c_int32, C.c_float, C.c_float, vp, C.c_uint32]
         L.oracle_bvh_sah.restype = C.c_float
+        L.oracle_obvh_pack.argtypes = [vp, vp, C.c_uint32, C.c_uint32, vp, vp, vp]
+        L.oracle_obvh_pack.restype = C.c_int
+        L.oracle_obvh_intersect.argtypes = [vp, vp, vp, vp, vp, vp, vp, C.c_uint32, vp, vp]
+        L.oracle_obvh_intersect.restype = C.c_int
+        L.oracle_ray_unit_box.argtypes = [vp, vp, vp]
+        L.oracle_ray_triangle.argtypes = [vp, vp, vp, vp, vp, C.c_float, vp]
         L.oracle_minverse4.argtypes = [vp, vp]
         L.oracle_mmul4.argtypes = [vp, vp, vp]
         _lib = L
@@ -210,4 +216,50 @@ def mmul4(a, b):
     b = _f32(b).reshape(16)
     out = np.zeros(16, dtype=np.float32)
     lib().oracle_mmul4(_p(a), _p(b), _p(out))
+    return out
+
+
+def obvh_pack(nodes, mats):
+    """obvh_persistent_prepare + kr_cuda_soa_obvh.  nodes: NODE_DTYPE (2N-1,
+    plain or collapsed), mats (2N-1, 16) float32.  Returns (left_T (N-1, 12),
+    right_T (N-1, 12), meta (2N-1, 4) int32)."""
+    nodes = np.ascontiguousarray(nodes, dtype=NODE_DTYPE)
+    total = nodes.shape[0]
+    internal = (total - 1) // 2
+    m = _f32(mats).reshape(total, 16)
+    L = np.zeros((internal, 12), dtype=np.float32)
+    R = np.zeros((internal, 12), dtype=np.float32)
+    meta = np.zeros((total, 4), dtype=np.int32)
+    if lib().oracle_obvh_pack(_p(nodes), _p(m), total, internal, _p(L), _p(R), _p(meta)) != 0:
+        raise ValueError("malformed tree")
+    return L, R, meta
+
+
+def obvh_intersect(L, R, meta, vertices, faces, primitives, rays, with_stats: bool = False):
+    """soa_obvh_persistent_intersect for every ray (RAY_DTYPE array).
+    Returns ISECT_DTYPE results (and per-ray stats: nodes visited, triangles
+    tested, deepest stack slot)."""
+    L, R = _f32(L), _f32(R)
+    meta = np.ascontiguousarray(meta, dtype=np.int32)
+    V = _f32(vertices)
+    F = np.ascontiguousarray(faces, dtype=np.uint32)
+    P = np.ascontiguousarray(primitives if primitives is not None else np.zeros(1), dtype=np.uint32)
+    rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+    n = rays.shape[0]
+    out = np.zeros(n, dtype=ISECT_DTYPE)
+    st = np.zeros((n, 3), dtype=np.uint32)
+    lib().oracle_obvh_intersect(_p(L), _p(R), _p(meta), _p(V), _p(F), _p(P), _p(rays), n, _p(out), _p(st))
+    return (out, st) if with_stats else out
+
+
+def ray_unit_box(rd, ro):
+    out = np.zeros(2, dtype=np.float32)
+    lib().oracle_ray_unit_box(_p(_f32(rd)), _p(_f32(ro)), _p(out))
+    return out
+
+
+def ray_triangle(direction, origin, v1, v2, v3, tmax):
+    out = np.zeros(3, dtype=np.float32)
+    lib().oracle_ray_triangle(_p(_f32(direction)), _p(_f32(origin)), _p(_f32(v1)), _p(_f32(v2)), _p(_f32(v3)),
+                              C.c_float(tmax), _p(out))
     return out

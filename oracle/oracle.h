@@ -26,6 +26,9 @@
  *   collapse.c       BVH collapse and SAH, src/ads/common/cuda/
  *                    transforms.cu:18-211,1061-1125,
  *                    src/ads/common/transforms.cpp:250-303
+ *   traverse.c       OBVH packing and closest-hit traversal,
+ *                    src/ads/common/cuda/transforms.cu:1298-1449,
+ *                    src/ads/common/cuda/intersectors.cu:801-972
  */
 #ifndef OBVH_ORACLE_H
 #define OBVH_ORACLE_H
@@ -106,6 +109,27 @@ int64_t oracle_collapse(const onode* nodes, const uint32_t* parents, uint32_t* c
 /* kr_bvh_sah over `total` nodes; obbs nullable (AABB areas). */
 float oracle_bvh_sah(const onode* nodes, int32_t root_idx, float ci, float ct, const oobb* obbs,
                      uint32_t total);
+
+typedef struct { ov3 origin; float tmin; ov3 direction; float tmax; } oray;       /* 32 B, kr_ray */
+typedef struct { ov3 normal; float bx, by; uint32_t primitive, instance; } oisect; /* 28 B, kr_intersection */
+
+/* obvh_persistent_prepare + kr_cuda_soa_obvh: metadata (4 int32 per node,
+ * node_count), left_T / right_T (12 floats per internal slot).  Returns -1 on
+ * an internal node outside [0, internal_count) or a child out of range. */
+int oracle_obvh_pack(const onode* nodes, const omat4* mats, uint32_t node_count, uint32_t internal_count,
+                     float* left_T, float* right_T, int32_t* meta);
+
+/* soa_obvh_persistent_intersect, one ray after another.  stats (nullable):
+ * per ray {internal nodes visited, triangles tested, deepest stack slot used
+ * (0xFFFFFFFF on overflow)}.  Returns -1 if a ray overflowed the stack. */
+int oracle_obvh_intersect(const float* left_T, const float* right_T, const int32_t* meta, const float* V,
+                          const uint32_t* F, const uint32_t* prims, const oray* rays, uint32_t n, oisect* out,
+                          uint32_t* stats);
+
+/* kr_ray_unit_aabb_intersect -> (t0, t1); kr_ray_triangle_intersect -> (b1, b2, t). */
+void oracle_ray_unit_box(const float* rd, const float* ro, float* out2);
+void oracle_ray_triangle(const float* dir, const float* org, const float* v1, const float* v2, const float* v3,
+                         float tmax, float* out3);
 
 /* 4x4 helpers exposed for tests (vecmath.h:946-1075,1214-1234). */
 void oracle_minverse4(const omat4* m, omat4* out);

@@ -1,0 +1,291 @@
+"""Pins of the OBVH packing and traversal oracle (oracle/traverse.c):
+
+* the unit-box and triangle tests against closed forms;
+* the packed layout against the tree and the transforms, element by element;
+* closest hits against an independent numpy brute force over every triangle
+  (the traversal can only miss a triangle the OBBs cull, never find a nearer
+  one, and on these meshes it finds the nearest for all but a handful of
+  grazing rays);
+* the depth-first walk against a Python restatement of the reference's
+  two-loop traversal with a postponed leaf (same hits, same tie-breaks);
+* collapsed and plain trees give the same hits;
+* stack use beyond 24 and exactly at 63 entries, and the overflow flag.
+"""
+import numpy as np
+import pytest
+
+from obvh_amd import synth
+from obvh_amd.layouts import ISECT_DTYPE, MISS, RAY_DTYPE, STACK_OVERFLOW
+
+SENTINEL = 0x76543210
+
+
+# ------------------------------------------------------------ primitives --
+
+def test_unit_box_closed_forms(oracle_lib):
+    ub = oracle_lib.ray_unit_box
+    # axis ray through the centre: enters at z=-0.5, leaves at z=0.5
+    assert list(ub([0, 0, 1], [0, 0, -2])) == [1.5, 2.5]
+    assert list(ub([0, 0, -2], [0, 0, 2])) == [0.75, 1.25]  # scaled direction
+    # starts inside: t0 < 0 < t1
+    assert list(ub([1, 0, 0], [0, 0, 0])) == [-0.5, 0.5]
+    # misses sideways and behind return (1, -1)
+    assert list(ub([0, 0, 1], [2, 0, -2])) == [1.0, -1.0]
+    assert list(ub([0, 0, 1], [0, 0, 2])) == [1.0, -1.0]
+    # diagonal: enters where the last slab is entered
+    t = ub([1, 1, 1], [-1, -2, -1])
+    assert list(t) == [1.5, 1.5]
+
+
+def test_triangle_closed_forms(oracle_lib):
+    rt = oracle_lib.ray_triangle
+    v1, v2, v3 = [0, 0, 0], [1, 0, 0], [0, 1, 0]
+    assert list(rt([0, 0, -1], [0.25, 0.5, 2], v1, v2, v3, 100.0)) == [0.25, 0.5, 2.0]
+    # barycentrics are along e1 = v2 - v1 and e2 = v3 - v1
+    assert list(rt([0, 0, -1], [0.5, 0.25, 3], v1, v2, v3, 100.0)) == [0.5, 0.25, 3.0]
+    # outside (b1 + b2 > 1), behind (t < 0), beyond tmax: (0, 0, tmax)
+    assert list(rt([0, 0, -1], [0.75, 0.5, 2], v1, v2, v3, 100.0)) == [0.0, 0.0, 100.0]
+    assert list(rt([0, 0, 1], [0.25, 0.25, 2], v1, v2, v3, 100.0)) == [0.0, 0.0, 100.0]
+    assert list(rt([0, 0, -1], [0.25, 0.25, 2], v1, v2, v3, 1.5)) == [0.0, 0.0, 1.5]
+    # both faces hit (no culling)
+    assert list(rt([0, 0, 1], [0.25, 0.25, -2], v1, v2, v3, 100.0)) == [0.25, 0.25, 2.0]
+
+
+# ----------------------------------------------------------------- trees --
+
+def lbvh_tree(oracle_lib, kind="ico"):
+    V, F = synth.icosphere(16, seed=2) if kind == "ico" else synth.hairball(tubes=400, seed=5)
+    nodes, parents, counts, costs, _ = oracle_lib.lbvh_build(V, F)
+    obbs, mats = oracle_lib.obb_tree(nodes, parents, V, F)
+    return V, F, nodes, parents, counts, mats
+
+
+def test_pack_layout(oracle_lib):
+    V, F, nodes, parents, counts, mats = lbvh_tree(oracle_lib)
+    col = oracle_lib.collapse(nodes, parents, counts)
+    for tree in (nodes, col["collapsed"]):
+        L, R, meta = oracle_lib.obvh_pack(tree, mats)
+        total = tree.shape[0]
+        I = (total - 1) // 2
+        m = mats.reshape(total, 4, 4)  # column major: m[node, column, row]
+        for i in range(total):
+            n = tree[i]
+            if n["nprim"] == 0xFFFF:
+                assert (meta[i] == 0).all()
+                continue
+            assert meta[i, 2] == n["axis"] and meta[i, 3] == 0
+            if n["nprim"] == 0:
+                a, b = int(n["a"]), int(n["b"])
+                ea = ~a if tree[a]["nprim"] > 0 else a
+                eb = ~b if tree[b]["nprim"] > 0 else b
+                assert (meta[i, 0], meta[i, 1]) == (ea, eb)
+                assert np.array_equal(L[i].reshape(4, 3), m[a, :, :3])
+                assert np.array_equal(R[i].reshape(4, 3), m[b, :, :3])
+            else:
+                assert (meta[i, 0], meta[i, 1]) == (n["a"], n["a"] + n["nprim"])
+                if i < I:
+                    assert (L[i] == 0).all() and (R[i] == 0).all()
+
+
+# ------------------------------------------------------------ brute force --
+
+def brute_force(V, F, rays, chunk=64):
+    """Nearest accepted t over all triangles, float32 arithmetic in the
+    reference's order, numpy vectorised (independent of the C oracle)."""
+    f32 = np.float32
+    tri = V[F[:, :3].astype(np.int64)].astype(f32)  # (nf, 3, 3)
+    v1, v2, v3 = tri[:, 0], tri[:, 1], tri[:, 2]
+    e1, e2 = v2 - v1, v3 - v1
+
+    def cross(a, b):
+        return np.stack([a[..., 1] * b[..., 2] - a[..., 2] * b[..., 1],
+                         a[..., 2] * b[..., 0] - a[..., 0] * b[..., 2],
+                         a[..., 0] * b[..., 1] - a[..., 1] * b[..., 0]], axis=-1)
+
+    def dot(a, b):
+        return a[..., 0] * b[..., 0] + a[..., 1] * b[..., 1] + a[..., 2] * b[..., 2]
+
+    best = np.empty(rays.shape[0], dtype=f32)
+    for s in range(0, rays.shape[0], chunk):
+        r = rays[s:s + chunk]
+        d = r["direction"][:, None, :].astype(f32)
+        o = r["origin"][:, None, :].astype(f32)
+        tmax = r["tmax"][:, None].astype(f32)
+        s1 = cross(np.broadcast_to(d, (d.shape[0],) + e2.shape), e2[None])
+        with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+            invd = f32(1.0) / dot(s1, e1[None])
+            dd = o - v1[None]
+            b1 = dot(dd, s1) * invd
+            s2 = cross(dd, np.broadcast_to(e1[None], dd.shape))
+            b2 = dot(np.broadcast_to(d, s2.shape), s2) * invd
+            t = dot(e2[None], s2) * invd
+            rej = (b1 < 0) | (b1 > 1) | (b2 < 0) | (b1 + b2 > 1) | (t < 0) | (t > tmax)
+        t = np.where(rej | np.isnan(t), tmax, t)
+        best[s:s + chunk] = t.min(axis=1)
+    return best
+
+
+def hit_t(oracle_lib, V, F, rays, out):
+    """Recompute each hit's t from its primitive with the oracle's triangle test."""
+    t = np.full(rays.shape[0], np.float32(synth.RAY_TMAX), dtype=np.float32)
+    for i in np.nonzero(out["primitive"] != MISS)[0]:
+        f = F[out["primitive"][i]]
+        t[i] = oracle_lib.ray_triangle(rays[i]["direction"], rays[i]["origin"], V[f[0]], V[f[1]], V[f[2]],
+                                       rays[i]["tmax"])[2]
+    return t
+
+
+@pytest.mark.parametrize("kind,rays_kind", [("ico", "camera"), ("ico", "random"), ("hair", "random")])
+def test_closest_hit_matches_brute_force(oracle_lib, kind, rays_kind):
+    V, F, nodes, parents, counts, mats = lbvh_tree(oracle_lib, kind)
+    rays = synth.camera_rays(V, 48, 48) if rays_kind == "camera" else synth.random_rays(V, 1500, seed=7)
+    L, R, meta = oracle_lib.obvh_pack(nodes, mats)
+    out = oracle_lib.obvh_intersect(L, R, meta, V, F, None, rays)
+    assert (out["instance"] == 0).all()
+    t_bvh = hit_t(oracle_lib, V, F, rays, out)
+    t_bf = brute_force(V, F, rays)
+    assert (t_bvh >= t_bf).all()                  # never nearer than the true nearest
+    miss = np.count_nonzero(t_bvh != t_bf)
+    assert miss <= max(2, rays.shape[0] // 500), f"{miss} rays lost their nearest triangle"
+    assert np.count_nonzero(out["primitive"] != MISS) > rays.shape[0] // 10
+    # hit records: normal of the hit face, misses keep the direction
+    h = out["primitive"] != MISS
+    assert np.array_equal(out["normal"][~h], rays["direction"][~h])
+    assert (out["barys"][~h] == 0.5).all()
+    assert np.allclose(np.linalg.norm(out["normal"][h], axis=1), 1.0, atol=1e-6)
+
+
+# ------------------------------------------------- the reference's loops --
+
+def reference_loops(oracle_lib, L, R, meta, V, F, prims, ray):
+    """The reference's traversal restated with its two inner loops and the
+    postponed leaf (intersectors.cu:840-941), one ray."""
+    org, d = ray["origin"], ray["direction"]
+    hitT = np.float32(ray["tmax"])
+    hit, bar = MISS, (0.5, 0.5)
+    stack, sp = [SENTINEL] + [0] * 63, 0
+    internal, leaf = 0, 0
+
+    def box(M):
+        M = M.reshape(4, 3).astype(np.float32)
+        rd = (M[0] * d[0] + M[1] * d[1]) + M[2] * d[2]
+        ro = ((M[0] * org[0] + M[1] * org[1]) + M[2] * org[2]) + M[3]
+        return oracle_lib.ray_unit_box(rd, ro)
+
+    while internal != SENTINEL:
+        searching = True
+        while searching and internal >= 0 and internal != SENTINEL:
+            li, ri = box(L[internal]), box(R[internal])
+            tl = li[1] >= li[0] and li[1] >= 0 and li[0] <= hitT
+            tr = ri[1] >= ri[0] and ri[1] >= 0 and ri[0] <= hitT
+            nxt, other = int(meta[internal, 0]), int(meta[internal, 1])
+            internal = nxt
+            if tl != tr:
+                if tr:
+                    internal = other
+            elif not tl:
+                internal = stack[sp]
+                sp -= 1
+            else:
+                if ri[0] < li[0]:
+                    internal, other = other, internal
+                sp += 1
+                stack[sp] = other
+            if internal < 0 and leaf >= 0:
+                searching = False
+                leaf = internal
+                internal = stack[sp]
+                sp -= 1
+        while leaf < 0:
+            beg, end = int(meta[~leaf, 0]), int(meta[~leaf, 1])
+            for k in range(beg, end):
+                pid = k if end - beg == 1 else int(prims[k])
+                f = F[pid]
+                b1, b2, t = oracle_lib.ray_triangle(d, org, V[f[0]], V[f[1]], V[f[2]], hitT)
+                if t < hitT:
+                    hitT, hit, bar = np.float32(t), pid, (b1, b2)
+            leaf = internal
+            if internal < 0:
+                internal = stack[sp]
+                sp -= 1
+    return hit, bar
+
+
+def test_depth_first_walk_equals_reference_loops(oracle_lib):
+    V, F = synth.icosphere(6, seed=9)
+    F = np.concatenate([F, F[::7]])  # duplicated faces: equal t, the visit order decides
+    nodes, parents, counts, costs, _ = oracle_lib.lbvh_build(V, F)
+    obbs, mats = oracle_lib.obb_tree(nodes, parents, V, F)
+    col = oracle_lib.collapse(nodes, parents, counts)
+    rays = np.concatenate([synth.camera_rays(V, 12, 12), synth.random_rays(V, 150, seed=3)])
+    for tree, prims in ((nodes, None), (col["collapsed"], col["primitives"])):
+        L, R, meta = oracle_lib.obvh_pack(tree, mats)
+        out = oracle_lib.obvh_intersect(L, R, meta, V, F, prims, rays)
+        for i in range(rays.shape[0]):
+            hit, bar = reference_loops(oracle_lib, L, R, meta, V, F, prims, rays[i])
+            assert out["primitive"][i] == hit
+            if hit != MISS:
+                assert tuple(out["barys"][i]) == tuple(np.float32(b) for b in bar)
+
+
+def test_collapsed_tree_same_hits(oracle_lib):
+    V, F, nodes, parents, counts, mats = lbvh_tree(oracle_lib, "hair")
+    col = oracle_lib.collapse(nodes, parents, counts)
+    rays = synth.random_rays(V, 2000, seed=12)
+    a = oracle_lib.obvh_intersect(*oracle_lib.obvh_pack(nodes, mats), V, F, None, rays)
+    b = oracle_lib.obvh_intersect(*oracle_lib.obvh_pack(col["collapsed"], mats), V, F, col["primitives"], rays)
+    same = np.count_nonzero(a["primitive"] == b["primitive"])
+    assert same >= rays.shape[0] - 2
+
+
+# ------------------------------------------------------------ deep stacks --
+
+def chain_tree(depth):
+    """Internal nodes 0..depth-1 in a chain down the left side; node i's right
+    child is leaf depth+i, the last node's left child leaf 2*depth.  Leaf
+    depth+k holds face k, a small triangle at z = k.  Every matrix maps the
+    box [-S/2, S/2]^3 onto the unit cube, so both children are always
+    entered and every level pushes its right leaf: the walk needs `depth`
+    stack slots."""
+    S = 4.0 * (depth + 2)
+    M = np.zeros(12, dtype=np.float32)
+    M[0] = M[4] = M[8] = 1.0 / S
+    L = np.tile(M, (depth, 1))
+    R = L.copy()
+    meta = np.zeros((2 * depth + 1, 4), dtype=np.int32)
+    for i in range(depth):
+        meta[i, 0] = i + 1 if i + 1 < depth else ~(2 * depth)
+        meta[i, 1] = ~(depth + i)
+    for k in range(depth + 1):
+        meta[depth + k, 0], meta[depth + k, 1] = k, k + 1
+    V = np.zeros((3 * (depth + 1), 3), dtype=np.float32)
+    F = np.zeros((depth + 1, 4), dtype=np.uint32)
+    for k in range(depth + 1):
+        V[3 * k:3 * k + 3] = [[-1, -1, k], [1, -1, k], [0, 1, k]]
+        F[k, :3] = [3 * k, 3 * k + 1, 3 * k + 2]
+    return L, R, meta, V, F
+
+
+def chain_rays():
+    r = np.zeros(3, dtype=RAY_DTYPE)
+    r["origin"] = [[0, 0, -1.5], [0, 0, 1e3], [5, 5, -1.5]]  # hits face 0 / hits the last face / misses all
+    r["direction"] = [[0, 0, 1], [0, 0, -1], [0, 0, 1]]
+    r["tmax"] = synth.RAY_TMAX
+    return r
+
+
+@pytest.mark.parametrize("depth", [40, 63])
+def test_deep_stack(oracle_lib, depth):
+    L, R, meta, V, F = chain_tree(depth)
+    out, st = oracle_lib.obvh_intersect(L, R, meta, V, F, None, chain_rays(), with_stats=True)
+    assert list(st[:, 2]) == [depth] * 3
+    assert list(out["primitive"]) == [0, depth, MISS]
+    assert (out["instance"] == 0).all()
+
+
+def test_stack_overflow_flag(oracle_lib):
+    L, R, meta, V, F = chain_tree(64)
+    out, st = oracle_lib.obvh_intersect(L, R, meta, V, F, None, chain_rays(), with_stats=True)
+    assert (out["instance"] == STACK_OVERFLOW).all() and (st[:, 2] == 0xFFFFFFFF).all()
+    assert (out["primitive"] == MISS).all()  # stops before the first leaf
+    assert out.dtype == ISECT_DTYPE
