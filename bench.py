@@ -13,6 +13,10 @@ roofline = algorithmic bytes of one transform call
 cpu_baseline = the CPU oracle (a single-thread C port of the same method) on
            the same tree, timed on this host; its output is also compared
            bit-exactly with the GPU output ("parity").
+Further legs (reported in the same JSON line): collapse + SAH of the tree,
+OBVH packing and closest-hit traversal of 1M camera and 1M random rays
+through the collapsed OBB tree, 2M-triangle transforms (LBVH and ATRBVH
+inputs), batched DiTO, and the serving loop.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -190,7 +194,79 @@ def serving_leg(args) -> dict:
     return out
 
 
-def transform_leg(V, F, label: str, steps: int, warmup: int, with_cpu: bool, builder: str = "lbvh") -> dict:
+def traversal_leg(V, F, Vd, Fd, nodes, mats, prims, label: str, steps: int, warmup: int, with_cpu: bool,
+                  width: int = 1024) -> dict:
+    """Pack the (collapsed) OBB tree and trace closest-hit queries through it:
+    a width x width pinhole camera (coherent) and as many random rays from a
+    sphere around the mesh (incoherent).  Per batch: device time, Mrays/s;
+    the CPU oracle re-traces a sample of every 16th ray (bit-exact parity and
+    single-thread CPU Mrays/s)."""
+    import numpy as np
+    import torch
+
+    from obvh_amd import ops, synth
+
+    ops.soa_pack(nodes, mats)
+    torch.cuda.synchronize()
+    p0, p1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    p0.record()
+    packed = ops.soa_pack(nodes, mats)
+    p1.record()
+    torch.cuda.synchronize()
+    pack_ms = p0.elapsed_time(p1)
+    out = {"pack_ms": round(pack_ms, 5), "batches": {}}
+    nh = L = R = M = ph = None
+    for kind in ("camera", "random"):
+        rays = synth.camera_rays(V, width, width) if kind == "camera" else synth.random_rays(V, width * width)
+        n = rays.shape[0]
+        rd = torch.from_numpy(rays.view(np.uint8)).cuda()
+        res = torch.empty(n * 28, dtype=torch.uint8, device="cuda")
+        ws = torch.empty(ops.lib().obvh_soa_intersect_workspace_size(n), dtype=torch.uint8, device="cuda")
+        for _ in range(warmup):
+            ops.soa_intersect(packed, Vd, Fd, rd, primitives=prims, workspace=ws, out=res)
+        torch.cuda.synchronize()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        t0 = time.perf_counter()
+        for k in range(steps):
+            evs[k][0].record()
+            ops.soa_intersect(packed, Vd, Fd, rd, primitives=prims, workspace=ws, out=res)
+            evs[k][1].record()
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) * 1e3 / steps
+        dev = sum(a.elapsed_time(b) for a, b in evs) / steps
+        got = ops.isects_to_numpy(res)
+        rec = {"rays": n, "ms_per_batch": round(wall, 5), "device_ms": round(dev, 5),
+               "Mrays_per_s": round(n / (wall * 1e-3) / 1e6, 2),
+               "hit_fraction": round(float(np.count_nonzero(got["primitive"] != 0xFFFFFFFF)) / n, 4),
+               "aborted": int(np.count_nonzero(got["instance"] != 0))}
+        if with_cpu:
+            import oracle
+
+            if nh is None:
+                nh = ops.nodes_to_numpy(nodes)
+                L, R, M = oracle.obvh_pack(nh, mats.cpu().numpy())
+                ph = None if prims is None else prims.cpu().numpy().view(np.uint32)
+            sample = rays[::16]
+            tc = time.perf_counter()
+            exp, st = oracle.obvh_intersect(L, R, M, V, F, ph, sample, with_stats=True)
+            cpu_s = time.perf_counter() - tc
+            g = got[::16]
+            rec["parity"] = bool(np.array_equal(g.view(np.uint8), exp.view(np.uint8)))
+            rec["nodes_per_ray"] = round(float(st[:, 0].mean()), 2)
+            rec["triangles_per_ray"] = round(float(st[:, 1].mean()), 2)
+            rec["max_stack"] = int(st[:, 2].max())
+            rec["cpu_baseline"] = {"value": round(sample.shape[0] / cpu_s / 1e6, 4), "unit": "Mrays/s", "cores": 1,
+                                   "kind": "port", "sample": f"every 16th ray ({sample.shape[0]} rays, {cpu_s:.2f} s)"}
+        print(f"[bench] trace {label} {kind}: {n} rays, {wall:.4f} ms wall, {dev:.4f} ms device, "
+              f"{rec['Mrays_per_s']} Mrays/s, hits {rec['hit_fraction']}, parity {rec.get('parity')}",
+              file=sys.stderr, flush=True)
+        out["batches"][kind] = rec
+        del rd, res, ws
+    return out
+
+
+def transform_leg(V, F, label: str, steps: int, warmup: int, with_cpu: bool, builder: str = "lbvh",
+                  trace: bool = False) -> dict:
     """LBVH (or ATRBVH: treelet 7, 2 passes) of the mesh built on the GPU, then
     `steps` timed transforms of the resident tree, then the collapse; optionally
     the CPU oracle on the same tree (bit-exact comparison and single-thread
@@ -279,6 +355,13 @@ def transform_leg(V, F, label: str, steps: int, warmup: int, with_cpu: bool, bui
             "sample": f"whole workload: one transform of the {N}-triangle tree ({cpu_s:.2f} s, single thread)",
         }
         print(f"[bench] cpu oracle {label}: {cpu_s:.2f}s, parity={out['parity']}", file=sys.stderr, flush=True)
+    if trace:
+        try:
+            out["traversal"] = traversal_leg(V, F, Vd, Fd, col["collapsed"], mats, col["primitives"], label,
+                                             max(5, steps // 4), 2, with_cpu)
+        except Exception as e:
+            print(f"[bench] traversal leg failed: {e!r}", file=sys.stderr, flush=True)
+            out["traversal"] = {"error": repr(e)}
     return out
 
 
@@ -294,6 +377,7 @@ def main() -> int:
     ap.add_argument("--no-plain", action="store_true", help="skip the one-call-per-set DiTO comparison")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-2m", action="store_true", help="skip the 2M-triangle transform legs")
+    ap.add_argument("--no-trace", action="store_true", help="skip the OBVH packing + traversal leg")
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="open-loop serving duration")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="open-loop arrival rate (req/s)")
     args = ap.parse_args()
@@ -311,7 +395,8 @@ def main() -> int:
     t_gen = time.perf_counter()
     V, F = synth.icosphere(args.freq, seed=2)
     print(f"[bench] device={dev}", file=sys.stderr, flush=True)
-    main_leg = transform_leg(V, F, f"icosphere f={args.freq}", args.steps, args.warmup, not args.no_cpu)
+    main_leg = transform_leg(V, F, f"icosphere f={args.freq}", args.steps, args.warmup, not args.no_cpu,
+                             trace=not args.no_trace)
     print(f"[bench] main workload ready and timed in {time.perf_counter() - t_gen:.1f}s", file=sys.stderr, flush=True)
     N, NV, internal = main_leg["triangles"], main_leg["vertices"], main_leg["internal_nodes"]
     ms_step, dev_ms_mean, value, achieved = (main_leg["ms_per_step"], main_leg["device_ms"],
@@ -381,6 +466,9 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
         "parity": parity,
+        "collapse_ms": main_leg["collapse_ms"],
+        "sah": main_leg["sah"],
+        "traversal": main_leg.get("traversal"),
         "transform_2m": extra,
         "dito": dito,
         "serving": served,
