@@ -228,45 +228,59 @@ def traversal_leg(V, F, Vd, Fd, nodes, mats, prims, label: str, steps: int, warm
                   width: int = 1024) -> dict:
     """Pack the (collapsed) OBB tree and trace closest-hit queries through it:
     a width x width pinhole camera (coherent) and as many random rays from a
-    sphere around the mesh (incoherent).  Per batch: device time, Mrays/s;
-    the CPU oracle re-traces a sample of every 16th ray (bit-exact parity and
-    single-thread CPU Mrays/s)."""
+    sphere around the mesh (incoherent).  Per batch: device time and Mrays/s
+    of the native record layout (the headline) and of the reference's SoA
+    layout (the drop-in), whose outputs must be identical; the CPU oracle
+    re-traces a sample of every 16th ray (bit-exact parity and single-thread
+    CPU Mrays/s)."""
     import numpy as np
     import torch
 
     from obvh_amd import ops, synth
 
     ops.soa_pack(nodes, mats)
+    ops.records_pack(nodes, mats, Vd, Fd, primitives=prims)
     torch.cuda.synchronize()
-    p0, p1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    p0, p1, p2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
     p0.record()
     packed = ops.soa_pack(nodes, mats)
     p1.record()
+    recs = ops.records_pack(nodes, mats, Vd, Fd, primitives=prims)
+    p2.record()
     torch.cuda.synchronize()
-    pack_ms = p0.elapsed_time(p1)
-    out = {"pack_ms": round(pack_ms, 5), "batches": {}}
+    out = {"pack_ms": round(p0.elapsed_time(p1), 5), "records_pack_ms": round(p1.elapsed_time(p2), 5),
+           "batches": {}}
     nh = L = R = M = ph = None
     for kind in ("camera", "random"):
         rays = synth.camera_rays(V, width, width) if kind == "camera" else synth.random_rays(V, width * width)
         n = rays.shape[0]
         rd = torch.from_numpy(rays.view(np.uint8)).cuda()
         res = torch.empty(n * 28, dtype=torch.uint8, device="cuda")
+        res_r = torch.empty(n * 28, dtype=torch.uint8, device="cuda")
         ws = torch.empty(ops.lib().obvh_soa_intersect_workspace_size(n), dtype=torch.uint8, device="cuda")
-        for _ in range(warmup):
-            ops.soa_intersect(packed, Vd, Fd, rd, primitives=prims, workspace=ws, out=res)
-        torch.cuda.synchronize()
-        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
-        t0 = time.perf_counter()
-        for k in range(steps):
-            evs[k][0].record()
-            ops.soa_intersect(packed, Vd, Fd, rd, primitives=prims, workspace=ws, out=res)
-            evs[k][1].record()
-        torch.cuda.synchronize()
-        wall = (time.perf_counter() - t0) * 1e3 / steps
-        dev = sum(a.elapsed_time(b) for a, b in evs) / steps
+
+        def timed(fn):
+            for _ in range(warmup):
+                fn()
+            torch.cuda.synchronize()
+            evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+                   for _ in range(steps)]
+            t0 = time.perf_counter()
+            for k in range(steps):
+                evs[k][0].record()
+                fn()
+                evs[k][1].record()
+            torch.cuda.synchronize()
+            return (time.perf_counter() - t0) * 1e3 / steps, sum(a.elapsed_time(b) for a, b in evs) / steps
+
+        wall_r, dev_r = timed(lambda: ops.records_intersect(recs, rd, workspace=ws, out=res_r))
+        wall, dev = timed(lambda: ops.soa_intersect(packed, Vd, Fd, rd, primitives=prims, workspace=ws, out=res))
         got = ops.isects_to_numpy(res)
-        rec = {"rays": n, "ms_per_batch": round(wall, 5), "device_ms": round(dev, 5),
-               "Mrays_per_s": round(n / (wall * 1e-3) / 1e6, 2),
+        rec = {"rays": n, "ms_per_batch": round(wall_r, 5), "device_ms": round(dev_r, 5),
+               "Mrays_per_s": round(n / (wall_r * 1e-3) / 1e6, 2),
+               "soa": {"ms_per_batch": round(wall, 5), "device_ms": round(dev, 5),
+                       "Mrays_per_s": round(n / (wall * 1e-3) / 1e6, 2)},
+               "records_equal_soa": bool(torch.equal(res, res_r)),
                "hit_fraction": round(float(np.count_nonzero(got["primitive"] != 0xFFFFFFFF)) / n, 4),
                "aborted": int(np.count_nonzero(got["instance"] != 0))}
         if with_cpu:
@@ -287,11 +301,12 @@ def traversal_leg(V, F, Vd, Fd, nodes, mats, prims, label: str, steps: int, warm
             rec["max_stack"] = int(st[:, 2].max())
             rec["cpu_baseline"] = {"value": round(sample.shape[0] / cpu_s / 1e6, 4), "unit": "Mrays/s", "cores": 1,
                                    "kind": "port", "sample": f"every 16th ray ({sample.shape[0]} rays, {cpu_s:.2f} s)"}
-        print(f"[bench] trace {label} {kind}: {n} rays, {wall:.4f} ms wall, {dev:.4f} ms device, "
-              f"{rec['Mrays_per_s']} Mrays/s, hits {rec['hit_fraction']}, parity {rec.get('parity')}",
-              file=sys.stderr, flush=True)
+        print(f"[bench] trace {label} {kind}: {n} rays, records {dev_r:.4f} ms device "
+              f"({rec['Mrays_per_s']} Mrays/s), SoA {dev:.4f} ms ({rec['soa']['Mrays_per_s']} Mrays/s), "
+              f"records == SoA {rec['records_equal_soa']}, hits {rec['hit_fraction']}, "
+              f"oracle parity {rec.get('parity')}", file=sys.stderr, flush=True)
         out["batches"][kind] = rec
-        del rd, res, ws
+        del rd, res, res_r, ws
     return out
 
 

@@ -95,6 +95,22 @@ typedef struct {
     u32 instance;    /* 0; 0xFFFFFFFF if the ray's traversal stack overflowed */
 } kr_intersection;                                         /* 28 B */
 
+/* Native OBVH layout (new): one 128-byte record (one cache line) per
+ * internal node holding both children's world -> unit-cube matrices and
+ * their codes -- an internal child's index, or for a leaf child
+ * ~(first_slot << 4 | count - 1) -- and one 48-byte record per triangle
+ * slot, a leaf's triangles in consecutive slots.                         */
+typedef struct {
+    kr_mat43 left_T, right_T;
+    int32_t left, right;
+    int32_t pad_[6];
+} obvh_node_record;                                        /* 128 B */
+typedef struct {
+    kr_vec3 v0, v1, v2;
+    u32 primitive;
+    u32 pad_[2];
+} obvh_tri_record;                                         /* 48 B */
+
 /* Collapse timings (ms), accumulated by kr_cuda_bvh_collapse. */
 typedef struct {
     kr_scalar collapse_mark_time;
@@ -319,6 +335,24 @@ kr_error obvh_soa_intersect_async(const kr_mat43* left_T, const kr_mat43* right_
                                   const kr_ray* rays, kr_intersection* isects, u32 ray_count,
                                   void* workspace, size_t workspace_bytes, void* stream);
 
+/* New: packs a (plain or collapsed) tree into the native record layout:
+ * node_records (face_count - 1 entries, one per internal node slot),
+ * tri_records (2 * face_count slots: a leaf node v >= face_count - 1 owns
+ * slot face_count + (v - face_count + 1), a collapsed leaf owns its slice
+ * of primitives).  A leaf may hold at most 16 primitives.  All pointers
+ * device, records 16-byte aligned; stream ordered.                       */
+kr_error obvh_records_pack_async(const kr_bvh_node* nodes, const kr_mat4* transforms, const u32* primitives,
+                                 const kr_vec3* vertices, u32 vertex_count, const kr_uvec4* faces,
+                                 u32 face_count, obvh_node_record* node_records,
+                                 obvh_tri_record* tri_records, void* stream);
+
+/* New: the closest-hit traversal over the record layout (same hits as the
+ * SoA traversal, bit for bit); workspace as obvh_soa_intersect_async.    */
+kr_error obvh_records_intersect_async(const obvh_node_record* node_records,
+                                      const obvh_tri_record* tri_records, u32 face_count,
+                                      const kr_ray* rays, kr_intersection* isects, u32 ray_count,
+                                      void* workspace, size_t workspace_bytes, void* stream);
+
 /* Last error string of the calling host thread (NULL if none). */
 const char* obvh_last_error(void);
 
@@ -339,6 +373,8 @@ static_assert(sizeof(kr_bvh_node_packed) == 64, "kr_bvh_node_packed must be 64 b
 static_assert(sizeof(kr_bvh_transformation_pair) == 128, "kr_bvh_transformation_pair must be 128 bytes");
 static_assert(sizeof(kr_ray) == 32, "kr_ray must be 32 bytes");
 static_assert(sizeof(kr_intersection) == 28, "kr_intersection must be 28 bytes");
+static_assert(sizeof(obvh_node_record) == 128, "obvh_node_record must be 128 bytes");
+static_assert(sizeof(obvh_tri_record) == 48, "obvh_tri_record must be 48 bytes");
 #endif
 
 #endif /* OBVH_OBVH_H */

@@ -37,6 +37,8 @@ EXPORTS = [
     "obvh_soa_pack_async",
     "obvh_soa_intersect_workspace_size",
     "obvh_soa_intersect_async",
+    "obvh_records_pack_async",
+    "obvh_records_intersect_async",
     "obvh_last_error",
     "obvh_version",
 ]
@@ -117,6 +119,10 @@ def lib():
     L.obvh_soa_intersect_workspace_size.restype = sz
     L.obvh_soa_intersect_async.argtypes = [vp, vp, vp, u32, u32, vp, u32, vp, vp, u32, vp, vp, u32, vp, sz, vp]
     L.obvh_soa_intersect_async.restype = C.c_char_p
+    L.obvh_records_pack_async.argtypes = [vp, vp, vp, vp, u32, vp, u32, vp, vp, vp]
+    L.obvh_records_pack_async.restype = C.c_char_p
+    L.obvh_records_intersect_async.argtypes = [vp, vp, u32, vp, vp, u32, vp, sz, vp]
+    L.obvh_records_intersect_async.restype = C.c_char_p
     L.obvh_last_error.argtypes = []
     L.obvh_last_error.restype = C.c_char_p
     L.obvh_version.argtypes = []

@@ -559,4 +559,24 @@ kr_error obvh_soa_intersect_async(const kr_mat43* left_T, const kr_mat43* right_
     return nullptr;
 }
 
+kr_error obvh_records_pack_async(const kr_bvh_node* nodes, const kr_mat4* transforms, const u32* primitives,
+                                 const kr_vec3* vertices, u32 vertex_count, const kr_uvec4* faces, u32 face_count,
+                                 obvh_node_record* node_records, obvh_tri_record* tri_records, void* stream) {
+    if (!nodes || !transforms || !vertices || !faces || !node_records || !tri_records)
+        return fail("obvh_records_pack: NULL array");
+    TRY(obvh::rec_pack_launch(nodes, transforms, primitives, vertices, faces, face_count, vertex_count, node_records,
+                              tri_records, (hipStream_t)stream));
+    return nullptr;
+}
+
+kr_error obvh_records_intersect_async(const obvh_node_record* node_records, const obvh_tri_record* tri_records,
+                                      u32 face_count, const kr_ray* rays, kr_intersection* isects, u32 ray_count,
+                                      void* workspace, size_t workspace_bytes, void* stream) {
+    if (ray_count == 0) return nullptr;
+    if (!node_records || !tri_records || !rays || !isects) return fail("obvh_records_intersect: NULL argument");
+    TRY(obvh::rec_intersect_launch(node_records, tri_records, face_count, rays, isects, ray_count, workspace,
+                                   workspace_bytes, (hipStream_t)stream));
+    return nullptr;
+}
+
 }  // extern "C"

@@ -10,15 +10,22 @@
 // second step runs on the host, here it is a kernel.
 //
 // Traversal (soa_obvh_persistent_intersect, intersectors.cu:801-972).  One
-// ray per lane, a persistent grid whose waves take 64 consecutive rays at a
-// time from a counter.  Per ray: the reference's depth-first walk with one
-// postponed leaf -- descend through internal nodes (both children tested
-// against the unit cube in their own frames; one entered: go, none: pop,
-// both: enter the nearer by t0, left on ties, push the other) until a leaf
-// is found, then intersect that leaf's triangles and any leaves popped
-// right after it.  Per lane the node/leaf sequence, every floating-point
-// operation and the strict "t < hitT" update equal the reference's, so the
-// hit (primitive, barycentrics, normal) is bit-identical.
+// ray per lane, a persistent grid; a lane whose ray is finished waits until
+// a quarter of its wave is idle, then those lanes take consecutive rays from
+// a global counter (one atomic per refill), so long rays do not hold idle
+// lanes back.  Per ray: the reference's depth-first walk -- at an internal
+// node both children are tested against the unit cube in their own frames
+// (one entered: go, none: pop, both: enter the nearer by t0, left on ties,
+// push the other); a leaf's triangles are tested in slice order, then the
+// walk pops.  The reference runs the walk as two inner loops (descend to a
+// leaf, then intersect it and any leaves popped right after it); that gives
+// the same per-ray sequence of node visits and triangle tests as this
+// single loop in which every lane does one step -- one node or one
+// triangle -- per iteration, which keeps the 64 lanes of a wave in step
+// (two inner loops each run as long as their slowest lane).  Per lane every
+// floating-point operation and the strict "t < hitT" update equal the
+// reference's, so the hit (primitive, barycentrics, normal) is
+// bit-identical.
 //
 // Stack: slot 0 is the implicit sentinel, slots 1..24 live in LDS
 // (slot-major, [slot][lane]: every access is bank-conflict free), slots
@@ -189,21 +196,107 @@ __device__ __forceinline__ f3 vload(const kr_vec3* V, uint32_t i) {
     return f3{v.x, v.y, v.z};
 }
 
-struct TravArgs {
+// How a traversal step reads the tree.  Both views give the same per-ray
+// sequence of node visits and triangle tests with the same values, so the
+// same hits.
+//
+// SoaTree: the reference's kr_SoA_OBVH arrays (3x4 matrices per internal
+// node, metadata per node, leaves' primitives through faces and vertices).
+struct SoaTree {
     const float4* LT;
     const float4* RT;
     const int4* meta;
     const kr_vec3* V;
     const kr_uvec4* F;
     const uint32_t* prims;
-    const kr_ray* rays;
-    kr_intersection* out;
-    uint32_t n;
-    uint32_t max_steps;
     uint32_t internal_count;  // entries of LT / RT
     uint32_t node_count;      // entries of meta
     uint32_t face_count;      // entries of F (and of prims)
     uint32_t vertex_count;    // entries of V
+
+    __device__ bool node(int32_t cur, M43& L, M43& R, int32_t& left, int32_t& right) const {
+        if ((uint32_t)cur >= internal_count) return false;
+        const int4 md = meta[cur];
+        L = load_m43(LT, cur);
+        R = load_m43(RT, cur);
+        left = md.x;
+        right = md.y;
+        return true;
+    }
+    // Leaf code -> primitive cursor [k, end); one: a single primitive whose id is k.
+    __device__ bool leaf(int32_t code, int32_t& k, int32_t& end, bool& one) const {
+        const uint32_t lf = (uint32_t)~code;
+        if (lf >= node_count) return false;
+        const int4 md = meta[lf];
+        k = md.x;
+        end = md.y;
+        one = end - k == 1;
+        return !(k < 0 || end < k || (!one && end > k && (!prims || (uint32_t)end > face_count)));
+    }
+    __device__ bool tri(int32_t k, bool one, uint32_t& pid, f3& v0, f3& v1, f3& v2) const {
+        pid = one ? (uint32_t)k : prims[k];
+        if (pid >= face_count) return false;
+        const kr_uvec4 f = F[pid];
+        if (f.x >= vertex_count || f.y >= vertex_count || f.z >= vertex_count) return false;
+        v0 = vload(V, f.x);
+        v1 = vload(V, f.y);
+        v2 = vload(V, f.z);
+        return true;
+    }
+    __device__ void hit_vertices(uint32_t pid, int32_t, f3& va, f3& vb, f3& vc) const {
+        const kr_uvec4 f = F[pid];
+        va = vload(V, f.x);
+        vb = vload(V, f.y);
+        vc = vload(V, f.z);
+    }
+};
+
+// RecTree: node records (128 B, one cache line: both children's matrices
+// and child codes; a leaf child is ~(first triangle slot << 4 | count - 1))
+// and triangle records (48 B: three vertices and the primitive id, a leaf's
+// triangles in consecutive slots).
+struct RecTree {
+    const float4* rec;   // 8 float4 per internal node
+    const float4* tris;  // 3 float4 per slot
+    uint32_t internal_count, slot_count, face_count;
+
+    __device__ bool node(int32_t cur, M43& L, M43& R, int32_t& left, int32_t& right) const {
+        if ((uint32_t)cur >= internal_count) return false;
+        const float4* p = rec + 8 * (size_t)cur;
+        L = M43{p[0], p[1], p[2]};
+        R = M43{p[3], p[4], p[5]};
+        const float4 c = p[6];
+        left = __float_as_int(c.x);
+        right = __float_as_int(c.y);
+        return true;
+    }
+    __device__ bool leaf(int32_t code, int32_t& k, int32_t& end, bool& one) const {
+        const uint32_t x = (uint32_t)~code;
+        k = (int32_t)(x >> 4);
+        end = k + (int32_t)(x & 15u) + 1;
+        one = false;
+        return (uint32_t)end <= slot_count;
+    }
+    __device__ bool tri(int32_t k, bool, uint32_t& pid, f3& v0, f3& v1, f3& v2) const {
+        const float4* p = tris + 3 * (size_t)k;
+        const float4 a = p[0], b = p[1], c = p[2];
+        v0 = f3{a.x, a.y, a.z};
+        v1 = f3{a.w, b.x, b.y};
+        v2 = f3{b.z, b.w, c.x};
+        pid = __float_as_uint(c.y);
+        return pid < face_count;
+    }
+    __device__ void hit_vertices(uint32_t, int32_t slot, f3& va, f3& vb, f3& vc) const {
+        uint32_t pid;
+        tri(slot, false, pid, va, vb, vc);
+    }
+};
+
+struct TraceIO {
+    const kr_ray* rays;
+    kr_intersection* out;
+    uint32_t n;
+    uint32_t max_steps;
     uint32_t* counter;
     int32_t* deep;
 };
@@ -212,58 +305,77 @@ __global__ void trav_clear(uint32_t* counter) {
     if (threadIdx.x == 0) *counter = 0;
 }
 
-__global__ __launch_bounds__(kTravThreads) void soa_intersect(TravArgs a) {
+template <class Tree>
+__global__ __launch_bounds__(kTravThreads) void trace(Tree tree, TraceIO a) {
     __shared__ int32_t lstack[kTravLdsSlots * kTravThreads];
     const uint32_t tid = threadIdx.x;
     const size_t stride = (size_t)gridDim.x * kTravThreads;
     int32_t* deep = a.deep + (size_t)blockIdx.x * kTravThreads + tid;
     const int lane = lane_id();
+    const uint64_t below = (1ull << lane) - 1;
+
+    // Per-lane ray state.  A lane is busy while it holds an unfinished ray;
+    // idle lanes are refilled from the counter once kTravRefill of them
+    // wait (or none is busy).  cur >= 0: internal node to visit; cur < 0:
+    // a leaf whose primitives [k, end) remain to be tested.
+    uint32_t ray = 0, hit = 0, steps = 0;
+    int32_t hit_slot = 0;
+    bool busy = false, exhausted = false, aborted = false;
+    f3 org{0.0f, 0.0f, 0.0f}, dir{0.0f, 0.0f, 0.0f};
+    float hitT = 0.0f, bx = 0.0f, by = 0.0f;
+    int sp = 0;
+    int32_t cur = kTravSentinel, k = 0, end = 0;
+    bool one = false;
+
+    auto pop = [&]() -> int32_t {
+        if (sp == 0) return kTravSentinel;
+        const int s = sp--;
+        return s <= kTravLdsSlots ? lstack[(s - 1) * kTravThreads + tid]
+                                  : deep[(size_t)(s - kTravLdsSlots - 1) * stride];
+    };
 
     for (;;) {
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(a.counter, (uint32_t)kWave);
-        base = (uint32_t)__shfl((int)base, 0, kWave);
-        if (base >= a.n) break;  // wave-uniform
-        const uint32_t ray = base + (uint32_t)lane;
-        const bool live = ray < a.n;
-
-        f3 org{0.0f, 0.0f, 0.0f}, dir{0.0f, 0.0f, 0.0f};
-        float hitT = 0.0f;
-        if (live) {
-            const kr_ray r = a.rays[ray];
-            org = f3{r.origin.x, r.origin.y, r.origin.z};
-            dir = f3{r.direction.x, r.direction.y, r.direction.z};
-            hitT = r.tmax;
-        }
-        uint32_t hit = 0xFFFFFFFFu, steps = 0;
-        float bx = 0.0f, by = 0.0f;
-        bool aborted = false;
-        int sp = 0;
-        int32_t cur = live ? 0 : kTravSentinel;
-        int32_t leaf = 0;  // < 0: a leaf waiting to be intersected
-
-        auto pop = [&]() -> int32_t {
-            if (sp == 0) return kTravSentinel;
-            const int s = sp--;
-            return s <= kTravLdsSlots ? lstack[(s - 1) * kTravThreads + tid]
-                                      : deep[(size_t)(s - kTravLdsSlots - 1) * stride];
-        };
-
-        while (cur != kTravSentinel) {
-            // Descend until one leaf is found (postponed) or the stack is empty.
-            while (cur >= 0 && cur != kTravSentinel) {
-                if (++steps > a.max_steps || (uint32_t)cur >= a.internal_count) {
-                    aborted = true;
-                    cur = kTravSentinel;
-                    break;
+        const bool wants = !busy && !exhausted;
+        const uint64_t idle = __ballot(wants);
+        const int nidle = __popcll(idle);
+        if (nidle >= kTravRefill || (nidle > 0 && __ballot(busy) == 0)) {  // wave-uniform
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(a.counter, (uint32_t)nidle);
+            base = (uint32_t)__shfl((int)base, 0, kWave);
+            if (wants) {
+                const uint32_t r = base + (uint32_t)__popcll(idle & below);
+                if (r < a.n) {
+                    const kr_ray q = a.rays[r];
+                    ray = r;
+                    busy = true;
+                    org = f3{q.origin.x, q.origin.y, q.origin.z};
+                    dir = f3{q.direction.x, q.direction.y, q.direction.z};
+                    hitT = q.tmax;
+                    hit = 0xFFFFFFFFu;
+                    steps = 0;
+                    bx = by = 0.0f;
+                    aborted = false;
+                    sp = 0;
+                    cur = 0;
+                } else {
+                    exhausted = true;
                 }
-                const int4 md = a.meta[cur];
-                const M43 L = load_m43(a.LT, cur);
-                const M43 R = load_m43(a.RT, cur);
+            }
+        }
+        if (__ballot(busy) == 0) break;  // every lane exhausted
+
+        // One step per lane: an internal node or one primitive of a leaf.
+        bool enter_leaf = false;
+        if (cur >= 0 && cur != kTravSentinel) {
+            M43 L, R;
+            int32_t next, other;
+            if (++steps > a.max_steps || !tree.node(cur, L, R, next, other)) {
+                aborted = true;
+                cur = kTravSentinel;
+            } else {
                 float tl, tr;
                 const bool go_l = enter_unit_box(m43_dir(L, dir), m43_point(L, org), hitT, &tl);
                 const bool go_r = enter_unit_box(m43_dir(R, dir), m43_point(R, org), hitT, &tr);
-                int32_t next = md.x, other = md.y;
                 if (go_l != go_r) {
                     if (go_r) next = other;
                 } else if (!go_l) {
@@ -276,59 +388,51 @@ __global__ __launch_bounds__(kTravThreads) void soa_intersect(TravArgs a) {
                     }
                     if (sp == kTravSlots) {
                         aborted = true;
-                        cur = kTravSentinel;
-                        break;
+                        next = kTravSentinel;
+                    } else {
+                        const int s = ++sp;
+                        if (s <= kTravLdsSlots) lstack[(s - 1) * kTravThreads + tid] = other;
+                        else deep[(size_t)(s - kTravLdsSlots - 1) * stride] = other;
                     }
-                    const int s = ++sp;
-                    if (s <= kTravLdsSlots) lstack[(s - 1) * kTravThreads + tid] = other;
-                    else deep[(size_t)(s - kTravLdsSlots - 1) * stride] = other;
                 }
                 cur = next;
-                if (cur < 0) {
-                    leaf = cur;
-                    cur = pop();
-                    break;
-                }
+                enter_leaf = cur < 0;
             }
-            // Intersect the postponed leaf and the leaves popped right after it.
-            while (leaf < 0) {
-                const uint32_t lf = (uint32_t)~leaf;
-                bool bad = ++steps > a.max_steps || lf >= a.node_count;
-                const int4 md = bad ? make_int4(0, 0, 0, 0) : a.meta[lf];
-                const int32_t beg = md.x, end = md.y, num = end - beg;
-                bad = bad || beg < 0 || end < beg || (num > 1 && (!a.prims || (uint32_t)end > a.face_count));
-                for (int32_t k = beg; k < end && !bad; ++k) {
-                    const uint32_t pid = (num == 1) ? (uint32_t)k : a.prims[k];
-                    if (pid >= a.face_count) {
-                        bad = true;
-                        break;
-                    }
-                    const kr_uvec4 f = a.F[pid];
-                    if (f.x >= a.vertex_count || f.y >= a.vertex_count || f.z >= a.vertex_count) {
-                        bad = true;
-                        break;
-                    }
+        } else if (cur < 0) {
+            bool bad = false;
+            if (k < end) {
+                uint32_t pid;
+                f3 v0, v1, v2;
+                bad = !tree.tri(k, one, pid, v0, v1, v2);
+                if (!bad) {
                     float b1 = 0.0f, b2 = 0.0f;
-                    const float t = ray_triangle(dir, org, vload(a.V, f.x), vload(a.V, f.y), vload(a.V, f.z), hitT,
-                                                 &b1, &b2);
+                    const float t = ray_triangle(dir, org, v0, v1, v2, hitT, &b1, &b2);
                     if (t < hitT) {
                         hitT = t;
                         hit = pid;
+                        hit_slot = k;
                         bx = b1;
                         by = b2;
                     }
                 }
-                if (bad) {
-                    aborted = true;
-                    cur = kTravSentinel;
-                    break;
-                }
-                leaf = cur;
-                if (cur < 0) cur = pop();
+                ++k;
+            }
+            if (bad) {
+                aborted = true;
+                cur = kTravSentinel;
+            } else if (k >= end) {
+                cur = pop();
+                enter_leaf = cur < 0;
+            }
+        }
+        if (enter_leaf) {  // the new current node is a leaf: its primitive cursor
+            if (++steps > a.max_steps || !tree.leaf(cur, k, end, one)) {
+                aborted = true;
+                cur = kTravSentinel;
             }
         }
 
-        if (live) {
+        if (busy && cur == kTravSentinel) {  // ray finished: write its record, free the lane
             kr_intersection o;
             o.instance = aborted ? kTravAborted : 0u;
             o.primitive = hit;
@@ -336,14 +440,92 @@ __global__ __launch_bounds__(kTravThreads) void soa_intersect(TravArgs a) {
                 o.geom_normal = kr_vec3{dir.x, dir.y, dir.z};
                 o.barys = kr_vec2{0.5f, 0.5f};
             } else {
-                const kr_uvec4 f = a.F[hit];
-                const f3 va = vload(a.V, f.x), vb = vload(a.V, f.y), vc = vload(a.V, f.z);
+                f3 va, vb, vc;
+                tree.hit_vertices(hit, hit_slot, va, vb, vc);
                 const f3 nrm = vm::normalize(vm::cross(sub3(vb, va), sub3(vc, va)));
                 o.geom_normal = kr_vec3{nrm.x, nrm.y, nrm.z};
                 o.barys = kr_vec2{bx, by};
             }
             a.out[ray] = o;
+            busy = false;
         }
+    }
+}
+
+// ----------------------------------------------------- record packing --
+//
+// One lane per node.  An internal node writes its record: the children's
+// 3x4 matrices and codes (a leaf child: ~(slot << 4 | count - 1)); a leaf
+// writes its triangles' records into its slots.  Slots: a leaf node v >= I
+// (I = N - 1 internal nodes) owns slot N + (v - I); a collapsed leaf
+// (v < I, 2..16 primitives) owns its slice of `prims`, slots
+// [primitivesOffset, primitivesOffset + count).  A leaf that cannot be
+// encoded (more than 16 primitives) gets an out-of-range code, a primitive
+// id out of range gets 0xFFFFFFFF: the traversal stops such rays.
+
+__device__ __forceinline__ int32_t rec_child(const kr_bvh_node* nodes, uint32_t c, uint32_t I, uint32_t N) {
+    const kr_bvh_node n = nodes[c];
+    if (n.nPrimitives == 0) return (int32_t)c;
+    const uint32_t cnt = n.nPrimitives;
+    if (cnt > 16 || (c < I && cnt < 2)) return 0x7FFFFFFF;
+    const uint32_t slot = c >= I ? N + (c - I) : n.u.primitivesOffset;
+    if (slot + cnt > 2 * N) return 0x7FFFFFFF;
+    return ~(int32_t)((slot << 4) | (cnt - 1));
+}
+
+__global__ __launch_bounds__(256) void rec_pack(const kr_bvh_node* __restrict__ nodes, const kr_mat4* __restrict__ mats,
+                                                const uint32_t* __restrict__ prims, const kr_vec3* __restrict__ V,
+                                                const kr_uvec4* __restrict__ F, uint32_t N, uint32_t vertex_count,
+                                                float4* __restrict__ rec, float4* __restrict__ tri) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t I = N - 1, total = 2 * N - 1;
+    if (v >= total) return;
+    const kr_bvh_node n = nodes[v];
+    if (n.nPrimitives == kCollapsedNode) return;
+    if (n.nPrimitives == 0) {
+        if (v >= I) return;
+        const uint32_t a = n.u.children.left, b = n.u.children.right;
+        float4 l[3] = {}, r[3] = {};
+        int32_t ca = 0x7FFFFFFF, cb = 0x7FFFFFFF;
+        if (a < total && b < total) {
+            m43_from_mat4(&mats[a], l);
+            m43_from_mat4(&mats[b], r);
+            ca = rec_child(nodes, a, I, N);
+            cb = rec_child(nodes, b, I, N);
+        }
+        float4* p = rec + 8 * (size_t)v;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            p[j] = l[j];
+            p[3 + j] = r[j];
+        }
+        p[6] = make_float4(__int_as_float(ca), __int_as_float(cb), 0.0f, 0.0f);
+        p[7] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
+    const uint32_t cnt = n.nPrimitives;
+    if (cnt > 16 || (v < I && cnt < 2)) return;
+    const uint32_t slot = v >= I ? N + (v - I) : n.u.primitivesOffset;
+    if (slot + cnt > 2 * N) return;
+    for (uint32_t j = 0; j < cnt; ++j) {
+        uint32_t pid = cnt == 1 ? n.u.primitivesOffset : (prims ? prims[n.u.primitivesOffset + j] : 0xFFFFFFFFu);
+        f3 v0{0, 0, 0}, v1 = v0, v2 = v0;
+        if (pid < N) {
+            const kr_uvec4 f = F[pid];
+            if (f.x < vertex_count && f.y < vertex_count && f.z < vertex_count) {
+                v0 = vload(V, f.x);
+                v1 = vload(V, f.y);
+                v2 = vload(V, f.z);
+            } else {
+                pid = 0xFFFFFFFFu;
+            }
+        } else {
+            pid = 0xFFFFFFFFu;
+        }
+        float4* p = tri + 3 * (size_t)(slot + j);
+        p[0] = make_float4(v0.x, v0.y, v0.z, v1.x);
+        p[1] = make_float4(v1.y, v1.z, v2.x, v2.y);
+        p[2] = make_float4(v2.z, __uint_as_float(pid), 0.0f, 0.0f);
     }
 }
 
@@ -406,28 +588,56 @@ kr_error intersect_launch(const kr_mat43* left_T, const kr_mat43* right_T, const
     if (ray_count == 0) return nullptr;
     if (!aligned16(left_T) || !aligned16(right_T) || !aligned16(meta))
         return "obvh_soa_intersect: left_T, right_T and metadata must be 16-byte aligned";
-    if (ray_count > 0xFFFFFFFFu - kWave) return "obvh_soa_intersect: too many rays";
+    if (ray_count > kTravMaxRays) return "obvh_soa_intersect: too many rays";
     TravWs ws;
     if (trav_carve(&ws, workspace, ws_bytes, ray_count) > ws_bytes || !workspace)
         return "obvh_soa_intersect: workspace too small";
-    TravArgs a{reinterpret_cast<const float4*>(left_T),
-               reinterpret_cast<const float4*>(right_T),
-               reinterpret_cast<const int4*>(meta),
-               V,
-               F,
-               prims,
-               rays,
-               out,
-               ray_count,
-               counts.max_steps,
-               counts.internal_count,
-               counts.node_count,
-               counts.face_count,
-               counts.vertex_count,
-               ws.counter,
-               ws.deep};
+    SoaTree tree{reinterpret_cast<const float4*>(left_T),
+                 reinterpret_cast<const float4*>(right_T),
+                 reinterpret_cast<const int4*>(meta),
+                 V,
+                 F,
+                 prims,
+                 counts.internal_count,
+                 counts.node_count,
+                 counts.face_count,
+                 counts.vertex_count};
+    TraceIO io{rays, out, ray_count, counts.max_steps, ws.counter, ws.deep};
     hipLaunchKernelGGL(trav_clear, dim3(1), dim3(64), 0, stream, ws.counter);
-    hipLaunchKernelGGL(soa_intersect, dim3(ws.blocks), dim3(kTravThreads), 0, stream, a);
+    hipLaunchKernelGGL(trace<SoaTree>, dim3(ws.blocks), dim3(kTravThreads), 0, stream, tree, io);
+    OBVH_HIP_TRY(hipGetLastError());
+    return nullptr;
+}
+
+kr_error rec_pack_launch(const kr_bvh_node* nodes, const kr_mat4* mats, const uint32_t* prims, const kr_vec3* V,
+                         const kr_uvec4* F, uint32_t face_count, uint32_t vertex_count, obvh_node_record* rec,
+                         obvh_tri_record* tri, hipStream_t stream) {
+    if (face_count < 2) return "obvh_records_pack: the tree needs at least two primitives";
+    if (face_count > (1u << 26)) return "obvh_records_pack: more than 2^26 primitives";
+    if (!aligned16(rec) || !aligned16(tri) || !aligned16(mats))
+        return "obvh_records_pack: records and transforms must be 16-byte aligned";
+    const uint32_t total = 2 * face_count - 1;
+    hipLaunchKernelGGL(rec_pack, dim3((total + 255) / 256), dim3(256), 0, stream, nodes, mats, prims, V, F, face_count,
+                       vertex_count, reinterpret_cast<float4*>(rec), reinterpret_cast<float4*>(tri));
+    OBVH_HIP_TRY(hipGetLastError());
+    return nullptr;
+}
+
+kr_error rec_intersect_launch(const obvh_node_record* rec, const obvh_tri_record* tri, uint32_t face_count,
+                              const kr_ray* rays, kr_intersection* out, uint32_t ray_count, void* workspace,
+                              size_t ws_bytes, hipStream_t stream) {
+    if (ray_count == 0) return nullptr;
+    if (face_count < 2 || face_count > (1u << 26)) return "obvh_records_intersect: face_count outside [2, 2^26]";
+    if (!aligned16(rec) || !aligned16(tri)) return "obvh_records_intersect: records must be 16-byte aligned";
+    if (ray_count > kTravMaxRays) return "obvh_records_intersect: too many rays";
+    TravWs ws;
+    if (trav_carve(&ws, workspace, ws_bytes, ray_count) > ws_bytes || !workspace)
+        return "obvh_records_intersect: workspace too small";
+    RecTree tree{reinterpret_cast<const float4*>(rec), reinterpret_cast<const float4*>(tri), face_count - 1,
+                 2 * face_count, face_count};
+    TraceIO io{rays, out, ray_count, 2 * face_count - 1, ws.counter, ws.deep};
+    hipLaunchKernelGGL(trav_clear, dim3(1), dim3(64), 0, stream, ws.counter);
+    hipLaunchKernelGGL(trace<RecTree>, dim3(ws.blocks), dim3(kTravThreads), 0, stream, tree, io);
     OBVH_HIP_TRY(hipGetLastError());
     return nullptr;
 }

@@ -14,6 +14,10 @@ constexpr int kTravSlots = 63;                 // slots 1..63 (slot 0 is the sen
 constexpr int kTravDeepSlots = kTravSlots - kTravLdsSlots;  // 25..63 in global memory
 constexpr uint32_t kTravMaxBlocks = 2048;      // persistent grid cap (256 CUs x 8)
 constexpr uint32_t kTravAborted = 0xFFFFFFFFu; // kr_intersection.instance of an aborted ray
+constexpr int kTravRefill = 16;                // idle lanes that trigger a refill of the wave
+// Every lane asks the counter for at most one ray past the end, so it never
+// exceeds ray_count + 64 x (waves of the grid); this bound keeps it in range.
+constexpr uint32_t kTravMaxRays = 0xFFFFFFFFu - (1u << 20);
 
 struct TravWs {
     uint32_t* counter;  // rays handed out so far
@@ -42,5 +46,11 @@ kr_error intersect_launch(const kr_mat43* left_T, const kr_mat43* right_T, const
                           const kr_vec3* V, const kr_uvec4* F, const uint32_t* prims, const kr_ray* rays,
                           kr_intersection* out, uint32_t ray_count, const TravCounts& counts, void* workspace,
                           size_t ws_bytes, hipStream_t stream);
+kr_error rec_pack_launch(const kr_bvh_node* nodes, const kr_mat4* mats, const uint32_t* prims, const kr_vec3* V,
+                         const kr_uvec4* F, uint32_t face_count, uint32_t vertex_count, obvh_node_record* rec,
+                         obvh_tri_record* tri, hipStream_t stream);
+kr_error rec_intersect_launch(const obvh_node_record* rec, const obvh_tri_record* tri, uint32_t face_count,
+                              const kr_ray* rays, kr_intersection* out, uint32_t ray_count, void* workspace,
+                              size_t ws_bytes, hipStream_t stream);
 
 }  // namespace obvh

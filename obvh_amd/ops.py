@@ -15,6 +15,9 @@ Mirrors the reference operator interface for the OBB path:
                                     intersectors.cuh:81-91).
 * ``soa_pack``, ``soa_intersect``   one-pass packing and the stream-ordered
                                     traversal with workspace.
+* ``records_pack``, ``records_intersect``
+                                    the native cache-line record layout and
+                                    its traversal (same hits).
 
 Device memory and streams come from torch (plumbing only); every compute step
 runs in libobvh.so.  Errors raise ObvhError; there is no CPU fallback.
@@ -286,6 +289,37 @@ def soa_intersect(packed, V, F, rays, primitives=None, stream=None, workspace=No
                                          packed["left_T"].shape[0], packed["meta"].shape[0], _ptr(V),
                                          V.numel() // 3, _ptr(F), _ptr(primitives), nf, _ptr(rays), _ptr(out), n,
                                          _ptr(ws), ws.numel(), _stream(stream)))
+    return out
+
+
+def records_pack(nodes, mats, V, F, primitives=None, stream=None):
+    """Native OBVH records of a device tree (plain or collapsed): dict(nodes
+    (uint8, 128 B per internal node), tris (uint8, 48 B per slot, 2N slots),
+    face_count)."""
+    torch = _torch()
+    total = nodes.numel() // NODE_DTYPE.itemsize
+    N = (total + 1) // 2
+    rec = torch.zeros(max(N - 1, 1) * 128, dtype=torch.uint8, device="cuda")
+    tri = torch.zeros(2 * N * 48, dtype=torch.uint8, device="cuda")
+    check(lib().obvh_records_pack_async(_ptr(nodes), _ptr(mats), _ptr(primitives), _ptr(V), V.numel() // 3,
+                                        _ptr(F), N, _ptr(rec), _ptr(tri), _stream(stream)))
+    return {"nodes": rec, "tris": tri, "face_count": N}
+
+
+def records_intersect(records, rays, stream=None, workspace=None, out=None):
+    """Closest hits against the record layout; same output as soa_intersect."""
+    torch = _torch()
+    if isinstance(rays, np.ndarray):
+        rays = torch.from_numpy(np.ascontiguousarray(rays, dtype=RAY_DTYPE).view(np.uint8)).cuda()
+    n = rays.numel() // RAY_DTYPE.itemsize
+    if out is None:
+        out = torch.empty(n * ISECT_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+    need = lib().obvh_soa_intersect_workspace_size(n)
+    ws = workspace if workspace is not None else _workspace(need)
+    if ws.numel() < need:
+        raise ValueError(f"workspace needs {need} bytes")
+    check(lib().obvh_records_intersect_async(_ptr(records["nodes"]), _ptr(records["tris"]), records["face_count"],
+                                             _ptr(rays), _ptr(out), n, _ptr(ws), ws.numel(), _stream(stream)))
     return out
 
 

@@ -79,6 +79,41 @@ def test_pack_and_intersect_match_oracle(torch, oracle_lib, kind, builder, colla
     got = ops.isects_to_numpy(out)
     same_isects(got, exp)
     assert np.count_nonzero(got["primitive"] != MISS) > rays.shape[0] // 10
+    # the native record layout: same records, byte for byte
+    recs = ops.records_pack(nodes, mats, Vd, Fd, primitives=prims_d)
+    out_r = ops.records_intersect(recs, rays)
+    torch.cuda.synchronize()
+    assert np.array_equal(out_r.cpu().numpy(), out.cpu().numpy())
+    check_records(recs, nh, eL, eR, em, V, F, prims_h)
+
+
+def check_records(recs, nh, eL, eR, em, V, F, prims_h):
+    """Decode the record layout and compare it with the oracle's SoA packing."""
+    N = recs["face_count"]
+    I = N - 1
+    R = recs["nodes"].cpu().numpy().view(np.float32).reshape(-1, 32)
+    T = recs["tris"].cpu().numpy().view(np.float32).reshape(-1, 12)
+    for i in range(I):
+        if nh[i]["nprim"] != 0:
+            continue
+        assert np.array_equal(R[i, 0:12].view(np.uint32), eL[i].view(np.uint32))
+        assert np.array_equal(R[i, 12:24].view(np.uint32), eR[i].view(np.uint32))
+        for side, code in ((0, R[i, 24:25].view(np.int32)[0]), (1, R[i, 25:26].view(np.int32)[0])):
+            exp_code = em[i, side]
+            if exp_code >= 0:
+                assert code == exp_code
+                continue
+            leaf = ~exp_code
+            beg, end = em[leaf, 0], em[leaf, 1]
+            pids = [beg] if end - beg == 1 else list(prims_h[beg:end])
+            x = ~int(code)
+            slot, cnt = x >> 4, (x & 15) + 1
+            assert cnt == len(pids)
+            for j, pid in enumerate(pids):
+                rec = T[slot + j]
+                assert rec[9:10].view(np.uint32)[0] == pid
+                f = F[pid]
+                assert np.array_equal(rec[0:9].reshape(3, 3), V[f[:3].astype(np.int64)])
 
 
 def test_legacy_path_matches_one_pass(torch, oracle_lib):
