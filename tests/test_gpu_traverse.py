@@ -88,14 +88,17 @@ def test_pack_and_intersect_match_oracle(torch, oracle_lib, kind, builder, colla
 
 
 def check_records(recs, nh, eL, eR, em, V, F, prims_h):
-    """Decode the record layout and compare it with the oracle's SoA packing."""
-    N = recs["face_count"]
-    I = N - 1
+    """Decode the record layout of every internal node reachable from the
+    root and compare it with the oracle's SoA packing.  (A collapsed tree
+    also holds unreachable internal nodes below a collapsed leaf.)"""
     R = recs["nodes"].cpu().numpy().view(np.float32).reshape(-1, 32)
     T = recs["tris"].cpu().numpy().view(np.float32).reshape(-1, 12)
-    for i in range(I):
+    stack = [0]
+    while stack:
+        i = stack.pop()
         if nh[i]["nprim"] != 0:
             continue
+        stack += [int(nh[i]["a"]), int(nh[i]["b"])]
         assert np.array_equal(R[i, 0:12].view(np.uint32), eL[i].view(np.uint32))
         assert np.array_equal(R[i, 12:24].view(np.uint32), eR[i].view(np.uint32))
         for side, code in ((0, R[i, 24:25].view(np.int32)[0]), (1, R[i, 25:26].view(np.int32)[0])):
