@@ -188,6 +188,24 @@ kr_error obvh_dito_batch(const float* x, const float* y, const float* z,
                          u32 tile_points, u32 workers,
                          void* workspace, size_t workspace_bytes, void* stream);
 
+/* Upload of nsets host point sets for obvh_dito_batch: set s is three rows
+ * (x, y, z) of offsets[s+1] - offsets[s] floats at host_sets[s] (pinned
+ * memory for an asynchronous copy); it lands at column offsets[s] of the
+ * device array soa with rows of row_stride floats (x = soa, y = soa +
+ * row_stride, z = soa + 2 row_stride).  offsets: host, nsets + 1 entries.
+ * One 2-D copy per set on `stream`.                                      */
+kr_error obvh_upload_sets(const float* const* host_sets, const kr_size* offsets, u32 nsets,
+                          float* soa, kr_size row_stride, void* stream);
+
+/* The same upload as one kernel that reads the sets over PCIe: host_sets
+ * and offsets are the host copies (checked: every non-empty set must be
+ * pinned host or device memory of 3 x its count floats), set_table and
+ * offsets_table the same arrays in device-accessible memory (device or
+ * pinned), which the kernel reads.                                       */
+kr_error obvh_gather_sets(const float* const* host_sets, const float* const* set_table,
+                          const kr_size* offsets, const kr_size* offsets_table, u32 nsets,
+                          float* soa, kr_size row_stride, void* stream);
+
 /* Transform with caller workspace, stream ordered.  Same semantics as
  * kr_cuda_bvh_obb_tree; `kdop_ids` (device, nullable) receives for every
  * node the 14 selected extremal vertex ids [min0..min6, max0..max6].      */

@@ -19,6 +19,8 @@ EXPORTS = [
     "obvh_dito_soa",
     "obvh_dito_batch_workspace_size",
     "obvh_dito_batch",
+    "obvh_upload_sets",
+    "obvh_gather_sets",
     "obvh_obb_tree_workspace_size",
     "obvh_obb_tree_async",
     "obvh_validate_tree_host",
@@ -82,6 +84,10 @@ def lib():
     L.obvh_dito_batch_workspace_size.restype = sz
     L.obvh_dito_batch.argtypes = [vp, vp, vp, vp, u32, u64, vp, vp, u32, u32, vp, sz, vp]
     L.obvh_dito_batch.restype = C.c_char_p
+    L.obvh_upload_sets.argtypes = [vp, vp, u32, vp, u64, vp]
+    L.obvh_upload_sets.restype = C.c_char_p
+    L.obvh_gather_sets.argtypes = [vp, vp, vp, vp, u32, vp, u64, vp]
+    L.obvh_gather_sets.restype = C.c_char_p
     L.obvh_obb_tree_workspace_size.argtypes = [u32, u32]
     L.obvh_obb_tree_workspace_size.restype = sz
     L.obvh_obb_tree_async.argtypes = [vp, vp, vp, vp, vp, vp, vp, u32, u32, vp, sz, vp]

@@ -24,7 +24,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libobvh.so")
 ARCH = os.environ.get("OBVH_ARCH", "gfx950")
 
-SOURCES = ["abi.hip", "dito.hip", "transform.hip", "lbvh.hip", "collapse.hip", "atrbvh.hip", "traverse.hip"]
+SOURCES = ["abi.hip", "dito.hip", "transform.hip", "lbvh.hip", "collapse.hip", "atrbvh.hip", "traverse.hip", "upload.hip"]
 
 FLAGS = [
     f"--offload-arch={ARCH}",

@@ -14,6 +14,7 @@
 #include "lbvh.hpp"
 #include "transform.hpp"
 #include "traverse.hpp"
+#include "upload.hpp"
 
 namespace {
 
@@ -119,6 +120,46 @@ kr_error obvh_dito_batch(const float* x, const float* y, const float* z, const k
     if (total_points > 0 && (!x || !y || !z)) return fail("obvh_dito_batch: NULL point array");
     TRY(obvh::dito_launch(x, y, z, nullptr, (const uint64_t*)offsets, 0, nsets, total_points, out_obbs, traces,
                           tile_points, workers, workspace, workspace_bytes, (hipStream_t)stream));
+    return nullptr;
+}
+
+// Upload of many host point sets into one SoA batch: one 2-D copy per set
+// (its x, y and z rows into the three rows of the (3, row_stride) array).
+kr_error obvh_upload_sets(const float* const* host_sets, const kr_size* offsets, u32 nsets, float* soa,
+                          kr_size row_stride, void* stream) {
+    if (nsets == 0) return nullptr;
+    if (!host_sets || !offsets || !soa) return fail("obvh_upload_sets: NULL argument");
+    if (offsets[nsets] > row_stride) return fail("obvh_upload_sets: offsets exceed the row stride");
+    for (u32 s = 0; s < nsets; ++s) {
+        if (offsets[s + 1] < offsets[s]) return fail("obvh_upload_sets: offsets must not decrease");
+        const size_t n = (size_t)(offsets[s + 1] - offsets[s]);
+        if (n == 0) continue;
+        if (!host_sets[s]) return fail("obvh_upload_sets: NULL set");
+        OBVH_HIP_TRY(hipMemcpy2DAsync(soa + offsets[s], row_stride * sizeof(float), host_sets[s], n * sizeof(float),
+                                      n * sizeof(float), 3, hipMemcpyHostToDevice, (hipStream_t)stream));
+    }
+    return nullptr;
+}
+
+kr_error obvh_gather_sets(const float* const* host_sets, const float* const* set_table, const kr_size* offsets,
+                          const kr_size* offsets_table, u32 nsets, float* soa, kr_size row_stride, void* stream) {
+    if (nsets == 0) return nullptr;
+    if (!host_sets || !set_table || !offsets || !offsets_table || !soa) return fail("obvh_gather_sets: NULL argument");
+    if (offsets[0] != 0) return fail("obvh_gather_sets: offsets[0] must be 0");
+    if (offsets[nsets] > row_stride) return fail("obvh_gather_sets: offsets exceed the row stride");
+    for (u32 s = 0; s < nsets; ++s) {
+        if (offsets[s + 1] < offsets[s]) return fail("obvh_gather_sets: offsets must not decrease");
+        if (offsets[s + 1] == offsets[s]) continue;
+        // the kernel reads the set over PCIe: it must be pinned (device-accessible) host memory
+        hipPointerAttribute_t attr;
+        if (!host_sets[s] || hipPointerGetAttributes(&attr, host_sets[s]) != hipSuccess ||
+            (attr.type != hipMemoryTypeHost && attr.type != hipMemoryTypeDevice)) {
+            (void)hipGetLastError();
+            return fail("obvh_gather_sets: a set is not pinned host or device memory");
+        }
+    }
+    TRY(obvh::gather_sets_launch(set_table, (const uint64_t*)offsets_table, nsets, soa, row_stride, offsets[nsets],
+                                 (hipStream_t)stream));
     return nullptr;
 }
 

@@ -11,9 +11,10 @@ Each scheduler iteration
   1. admits arrived requests in arrival order while their device footprint
      fits the HBM budget (inputs, outputs and workspaces stay resident until
      the result is back on the host);
-  2. packs every admitted fit into ONE batched DiTO launch (uploads, offsets,
-     obvh_dito_batch, download) on one of two alternating streams, so a new
-     batch starts while the previous one still runs;
+  2. packs every admitted fit into ONE batched DiTO launch (one gather
+     kernel reads every set from pinned host memory into the batch's SoA
+     array, then obvh_dito_batch, download) on one of two alternating
+     streams, so a new batch starts while the previous one still runs;
   3. starts each admitted tree request on the next of several streams
      (upload, obvh_lbvh_build, obvh_obb_tree_async, root box download);
   4. polls completion events and returns results.
@@ -23,6 +24,7 @@ a request's result is the same alone and in any mix.
 from __future__ import annotations
 
 import collections
+import ctypes
 import dataclasses
 import math
 import time
@@ -30,7 +32,7 @@ import time
 import numpy as np
 
 from . import ops
-from ._lib import lib
+from ._lib import check, lib
 from .layouts import NODE_DTYPE
 
 
@@ -117,22 +119,34 @@ class Server:
         self._streams()
         stream = self.fit_streams[self._fs]
         self._fs ^= 1
+        for r in fits:
+            p = r.points
+            if (p.dim() != 2 or p.shape[0] != 3 or p.dtype != torch.float32 or not p.is_contiguous()
+                    or not (p.is_pinned() or p.is_cuda)):
+                raise ValueError(f"request {r.rid}: points must be a contiguous float32 (3, n) tensor "
+                                 "in pinned host or device memory")
         sizes = np.array([int(r.points.shape[1]) for r in fits], dtype=np.int64)
         off = np.zeros(len(fits) + 1, dtype=np.int64)
         off[1:] = np.cumsum(sizes)
         total = int(off[-1])
         with torch.cuda.stream(stream):
             soa = torch.empty((3, max(total, 1)), dtype=torch.float32, device="cuda")
-            for r, a, b in zip(fits, off[:-1], off[1:]):
-                soa[:, int(a):int(b)].copy_(r.points, non_blocking=True)
+            addrs = [r.points.data_ptr() for r in fits]
+            ptrs = (ctypes.c_void_p * len(fits))(*addrs)
+            ptrs_h = torch.tensor(addrs, dtype=torch.int64).pin_memory()
             offs_h = torch.from_numpy(off).pin_memory()
+            ptrs_d = ptrs_h.to("cuda", non_blocking=True)
             offs = offs_h.to("cuda", non_blocking=True)
+            check(lib().obvh_gather_sets(ptrs, ctypes.c_void_p(ptrs_d.data_ptr()), off.ctypes.data_as(ctypes.c_void_p),
+                                         ctypes.c_void_p(offs.data_ptr()), len(fits),
+                                         ctypes.c_void_p(soa.data_ptr()), max(total, 1),
+                                         ctypes.c_void_p(stream.cuda_stream)))
             out = ops.dito_batch(soa[0], soa[1], soa[2], offs, stream=stream)
             host = torch.empty(out.shape, dtype=torch.float32, pin_memory=True)
             host.copy_(out, non_blocking=True)
             ev = torch.cuda.Event()
             ev.record(stream)
-        keep = (soa, offs, offs_h, out)
+        keep = (soa, offs, offs_h, ptrs_h, ptrs_d, out)
 
         def finish(now, fits=fits, host=host, keep=keep):
             res = host.numpy()
