@@ -508,7 +508,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": pmc_traffic("transform_icosphere224_pmc.md") if args.freq == 224 else None,
+            "traffic": pmc_traffic("transform_icosphere224_pmc.txt") if args.freq == 224 else None,
             "kernel": "obb_tree call (leaf_boxes, climb_leaves, climb_phase x2, candidates, refit, finalize)",
             "bytes_per_call": balg,
             "device_ms": round(dev_ms_mean, 5),
