@@ -1035,9 +1035,26 @@ __global__ __launch_bounds__(kTile) void refit(const kr_bvh_node* __restrict__ n
         }
     }
     __syncthreads();
-    // fold the tile's LDS keys into global memory
-    if (lo + t < I && (R.tk[t][0] != kKeyHi || R.tk[t][3] != kKeyLo))
-        fold_global(&ws.range[(size_t)(lo + t) * 6], R.tk[t]);
+    // fold the tile's LDS keys into global memory.  A node whose leaves all
+    // lie in this tile has no contributor outside this workgroup, and within
+    // it only the LDS slot (tier C, which folds with atomics, never takes
+    // such a node except the tile-common node): its keys are merged with a
+    // plain read-modify-write.  The others are merged with atomics.
+    if (lo + t < I && (R.tk[t][0] != kKeyHi || R.tk[t][3] != kKeyLo)) {
+        const uint32_t node = lo + t;
+        const uint2 sp = ws.span[node];
+        int32_t* rk = &ws.range[(size_t)node * 6];
+        const bool chain_node = R.nchain > 0 && R.chain[0] == node;
+        if (sp.x >= lo && span_hi(sp) <= thi && !chain_node) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                rk[a] = min(rk[a], R.tk[t][a]);
+                rk[3 + a] = max(rk[3 + a], R.tk[t][3 + a]);
+            }
+        } else {
+            fold_global(rk, R.tk[t]);
+        }
+    }
     if (t < kRHash && R.hkey[t] != kEmpty) fold_global(&ws.range[(size_t)R.hkey[t] * 6], R.hval[t]);
 }
 
