@@ -13,11 +13,12 @@
 //  atr_treelets   one optimiser pass: bottom-up over the current tree; a node
 //                 holding at least gamma triangles is optimised as a treelet
 //                 root right after its subtree (AgglomerativeTreelet.cu:
-//                 405-561, Treelet.cuh:55-127).  The wave of the lane that
-//                 completes the node forms the treelet (largest-area
-//                 expansion), builds the distance matrix (union box areas),
-//                 merges the closest pair ts-1 times and writes the new
-//                 topology when its SAH is lower.  Lanes 0..31 run the
+//                 405-561, Treelet.cuh:55-127).  A half-wave (32 lanes) of
+//                 the wave whose lane completes the node forms the treelet
+//                 (largest-area expansion), builds the distance matrix
+//                 (union box areas), merges the closest pair ts-1 times and
+//                 writes the new topology when its SAH is lower; the two
+//                 halves take two treelets at once.  Each half runs the
 //                 reference's 32-lane __shfl_down tournaments exactly, so
 //                 ties resolve as in the warp code.
 //  (final)        leaf bounds from the working boxes, then bounds / counts /
@@ -103,6 +104,9 @@ __global__ __launch_bounds__(256) void atr_scene(const kr_vec3* V, const kr_uvec
             }
         }
     }
+    // wave, then workgroup reduction; one filtered atomic per key and
+    // workgroup (a few thousand same-address atomics would serialise)
+    __shared__ int32_t part[256 / kWave][6];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         mn[a] = wave_min_i32(mn[a]);
@@ -111,8 +115,19 @@ __global__ __launch_bounds__(256) void atr_scene(const kr_vec3* V, const kr_uvec
     if (lane_id() == 0) {
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            atomicMin(&ws.sbox[a], mn[a]);
-            atomicMax(&ws.sbox[3 + a], mx[a]);
+            part[wave_id()][a] = mn[a];
+            part[wave_id()][3 + a] = mx[a];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int a = threadIdx.x;
+        int32_t v = part[0][a];
+        for (int w = 1; w < 256 / kWave; ++w) v = a < 3 ? min(v, part[w][a]) : max(v, part[w][a]);
+        const int32_t cur = __hip_atomic_load(&ws.sbox[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (a < 3 ? v < cur : v > cur) {
+            if (a < 3) atomicMin(&ws.sbox[a], v);
+            else atomicMax(&ws.sbox[a], v);
         }
     }
 }
@@ -255,13 +270,20 @@ __device__ __forceinline__ float4 shfl4(float4 v, int src) {
     return make_float4(__shfl(v.x, src), __shfl(v.y, src), __shfl(v.z, src), 0.0f);
 }
 
-// Optimise the treelet rooted at `root` with the whole wave (all lanes active).
-__device__ void treelet_wave(uint32_t root, kr_bvh_node* nodes, uint32_t* parents, const AtrWs& ws, uint32_t N,
-                             int ts, float ci, TreeletLds& S) {
-    const int lane = lane_id();
+// Optimise the treelet rooted at `root` with one half of the wave: lanes
+// 0-31 and 32-63 each take a treelet (`valid` false: that half idles), as
+// the reference's 32-lane warps do; every tournament is a width-32
+// shuffle-down within the half, so ties break as in the warp code.  All 64
+// lanes execute (the loops depend only on ts).  Two treelets found in the
+// same round of the climb have disjoint subtrees, so running them together
+// equals running them one after the other.
+__device__ void treelet_half(uint32_t root, bool valid, kr_bvh_node* nodes, uint32_t* parents, const AtrWs& ws,
+                             uint32_t N, int ts, float ci, TreeletLds& S) {
+    const int hl = lane_id() & 31;  // lane within the half
+    const int hb = lane_id() & 32;  // first lane of the half
     const uint32_t I = N - 1;
     // formTreelet: expand the treelet leaf with the largest area ts-2 times
-    if (lane == 0) {
+    if (valid && hl == 0) {
         const uint32_t L = nodes[root].u.children.left, R = nodes[root].u.children.right;
         S.tin[0] = root;
         S.tl[0] = L;
@@ -273,16 +295,16 @@ __device__ void treelet_wave(uint32_t root, kr_bvh_node* nodes, uint32_t* parent
     for (int it = 0; it < ts - 2; ++it) {
         int idx = -1;
         float a = -FLT_MAX;
-        if (lane < 2 + it) {
-            idx = lane;
-            a = S.tl[lane] >= I ? -FLT_MAX : S.ta[lane];
+        if (valid && hl < 2 + it) {
+            idx = hl;
+            a = S.tl[hl] >= I ? -FLT_MAX : S.ta[hl];
         }
         for (int n = 2 + it, amount = n / 2; n > 1; n = (n + 1) / 2, amount = n / 2) {
             const int oi = __shfl_down(idx, amount, 32);
             const float oa = __shfl_down(a, amount, 32);
             if (oa > a) { a = oa; idx = oi; }
         }
-        if (lane == 0) {
+        if (valid && hl == 0) {
             const uint32_t rep = S.tl[idx];
             const uint32_t l2 = nodes[rep].u.children.left, r2 = nodes[rep].u.children.right;
             S.tin[it + 1] = rep;
@@ -293,26 +315,27 @@ __device__ void treelet_wave(uint32_t root, kr_bvh_node* nodes, uint32_t* parent
         }
         wave_sync();
     }
-    // lane state: node, subtree SAH, box of treelet leaf `lane`
+    // lane state: node, subtree SAH, box of treelet leaf `hl`
     uint32_t tnode = 0;
     float tsah = 0.0f;
     float4 bmn = make_float4(0, 0, 0, 0), bmx = bmn;
-    if (lane < ts) {
-        tnode = S.tl[lane];
+    if (valid && hl < ts) {
+        tnode = S.tl[hl];
         tsah = ws.sah[tnode];
         bmn = ws.bmin[tnode];
         bmx = ws.bmax[tnode];
     }
     // distance matrix: union areas of every pair (row > col)
     const int npairs = ts * (ts - 1) / 2;
-    for (int base = 0; base < npairs; base += kWave) {
-        const int e = base + lane;
+    for (int base = 0; base < npairs; base += 32) {
+        const int e = base + hl;
         int r = 0, c = 0;
         if (e < npairs) {
             r = trm_row(e);
             c = e - trm_index(r, 0);
         }
-        const float4 amn = shfl4(bmn, r), amx = shfl4(bmx, r), cmn = shfl4(bmn, c), cmx = shfl4(bmx, c);
+        const float4 amn = shfl4(bmn, hb + r), amx = shfl4(bmx, hb + r), cmn = shfl4(bmn, hb + c),
+                     cmx = shfl4(bmx, hb + c);
         if (e < npairs) S.dist[e] = union_area(amn, amx, cmn, cmx);
     }
     wave_sync();
@@ -321,42 +344,41 @@ __device__ void treelet_wave(uint32_t root, kr_bvh_node* nodes, uint32_t* parent
         const int size = last * (1 + last) / 2;
         float md = FLT_MAX;
         int mi = 0;
-        if (lane < 32)
-            for (int j = lane; j < size; j += 32) {
-                const float d = S.dist[j];
-                if (d < md) { md = d; mi = j; }
-            }
+        for (int j = hl; j < size; j += 32) {
+            const float d = S.dist[j];
+            if (d < md) { md = d; mi = j; }
+        }
         for (int n = 32, amount = 16; n > 1; n = (n + 1) / 2, amount = n / 2) {
             const int oi = __shfl_down(mi, amount, 32);
             const float od = __shfl_down(md, amount, 32);
             if (od < md) { md = od; mi = oi; }
         }
-        const int min_index = __shfl(mi, 0);
+        const int min_index = __shfl(mi, hb);
         const int row = trm_row(min_index), col = min_index - trm_index(row, 0);
         // updateState
-        const float s = __shfl(tsah, row) + __shfl(tsah, col);
-        const uint32_t li = (uint32_t)__shfl((int)tnode, row), ri = (uint32_t)__shfl((int)tnode, col);
-        float4 amn = shfl4(bmn, row), amx = shfl4(bmx, row);
-        const float4 cmn = shfl4(bmn, col), cmx = shfl4(bmx, col);
+        const float s = __shfl(tsah, hb + row) + __shfl(tsah, hb + col);
+        const uint32_t li = (uint32_t)__shfl((int)tnode, hb + row), ri = (uint32_t)__shfl((int)tnode, hb + col);
+        float4 amn = shfl4(bmn, hb + row), amx = shfl4(bmx, hb + row);
+        const float4 cmn = shfl4(bmn, hb + col), cmx = shfl4(bmx, hb + col);
         amn = make_float4(zmin(cmn.x, amn.x), zmin(cmn.y, amn.y), zmin(cmn.z, amn.z), 0.0f);
         amx = make_float4(zmax(cmx.x, amx.x), zmax(cmx.y, amx.y), zmax(cmx.z, amx.z), 0.0f);
-        if (lane == col) {
+        if (hl == col) {
             tnode = S.tin[last - 1];
             bmn = amn;
             bmx = amx;
             S.ta[last] = s + ci * box_area(amn, amx);
             tsah = S.ta[last];
         }
-        const uint32_t last_node = (uint32_t)__shfl((int)tnode, last);
-        const float last_sah = __shfl(tsah, last);
-        const float4 lmn = shfl4(bmn, last), lmx = shfl4(bmx, last);
-        if (lane == row) {
+        const uint32_t last_node = (uint32_t)__shfl((int)tnode, hb + last);
+        const float last_sah = __shfl(tsah, hb + last);
+        const float4 lmn = shfl4(bmn, hb + last), lmx = shfl4(bmx, hb + last);
+        if (hl == row) {
             tnode = last_node;
             tsah = last_sah;
             bmn = lmn;
             bmx = lmx;
         }
-        if (lane == last) {
+        if (hl == last) {
             tnode = li;
             S.tl[last] = ri;
             bmn = amn;
@@ -365,32 +387,32 @@ __device__ void treelet_wave(uint32_t root, kr_bvh_node* nodes, uint32_t* parent
         wave_sync();
         // updateDistancesMatrix
         if (last > 1) {
-            if (lane < last && lane != row) {
-                const float v = S.dist[trm_index(last, lane)];
-                S.dist[trm_index(max(row, lane), min(row, lane))] = v;
+            if (hl < last && hl != row) {
+                const float v = S.dist[trm_index(last, hl)];
+                S.dist[trm_index(max(row, hl), min(row, hl))] = v;
             }
             wave_sync();
-            int dr = lane, dc = lane;
-            if (lane < last && lane != col) {
-                dr = max(col, lane);
-                dc = min(col, lane);
+            int dr = hl, dc = hl;
+            if (hl < last && hl != col) {
+                dr = max(col, hl);
+                dc = min(col, hl);
             }
-            const float4 xmn = shfl4(bmn, dr & 63), xmx = shfl4(bmx, dr & 63);
-            const float4 ymn = shfl4(bmn, dc & 63), ymx = shfl4(bmx, dc & 63);
-            if (lane < last && lane != col) S.dist[trm_index(dr, dc)] = union_area(xmn, xmx, ymn, ymx);
+            const float4 xmn = shfl4(bmn, hb + (dr & 31)), xmx = shfl4(bmx, hb + (dr & 31));
+            const float4 ymn = shfl4(bmn, hb + (dc & 31)), ymx = shfl4(bmx, hb + (dc & 31));
+            if (hl < last && hl != col) S.dist[trm_index(dr, dc)] = union_area(xmn, xmx, ymn, ymx);
         }
         wave_sync();
     }
     // updateTreelet: keep the new topology when its SAH is lower
-    if (S.ta[1] < ws.sah[S.tin[0]]) {
-        if (lane >= 1 && lane < ts) {
-            const uint32_t v = S.tin[lane - 1];
-            const uint32_t right = S.tl[lane];
+    if (valid && S.ta[1] < ws.sah[S.tin[0]]) {
+        if (hl >= 1 && hl < ts) {
+            const uint32_t v = S.tin[hl - 1];
+            const uint32_t right = S.tl[hl];
             nodes[v].u.children.left = tnode;
             nodes[v].u.children.right = right;
             parents[tnode] = v;
             parents[right] = v;
-            ws.sah[v] = S.ta[lane];
+            ws.sah[v] = S.ta[hl];
             ws.bmin[v] = bmn;
             ws.bmax[v] = bmx;
             ws.area[v] = box_area(bmn, bmx);
@@ -404,8 +426,8 @@ __device__ void treelet_wave(uint32_t root, kr_bvh_node* nodes, uint32_t* parent
 __global__ __launch_bounds__(1024) void atr_treelets(kr_bvh_node* nodes, uint32_t* parents, AtrWs ws, uint32_t nf,
                                                      int ts, int gamma, float ci, float ct, uint32_t phase,
                                                      uint32_t chunk) {
-    __shared__ TreeletLds lds[1024 / kWave];
-    TreeletLds& S = lds[wave_id()];
+    __shared__ TreeletLds lds[2 * 1024 / kWave];
+    TreeletLds& S = lds[2 * wave_id() + (lane_id() >> 5)];
     const uint32_t I = nf - 1;
     const uint32_t tag = (phase << 24) | (blockIdx.x + 1u);
     const uint64_t lo = (uint64_t)blockIdx.x * chunk, hi = lo + chunk;
@@ -482,10 +504,14 @@ __global__ __launch_bounds__(1024) void atr_treelets(kr_bvh_node* nodes, uint32_
                 treelet = tri >= gamma;
             }
             uint64_t vote = __ballot(treelet);
-            while (vote) {
-                const int k = __builtin_ctzll(vote);
-                treelet_wave((uint32_t)__shfl((int)node, k), nodes, parents, ws, nf, ts, ci, S);
+            while (vote) {  // two treelets at a time, one per half-wave
+                const int k1 = __builtin_ctzll(vote);
                 vote &= vote - 1;
+                const int k2 = vote ? __builtin_ctzll(vote) : k1;
+                if (vote) vote &= vote - 1;
+                const uint32_t n1 = (uint32_t)__shfl((int)node, k1), n2 = (uint32_t)__shfl((int)node, k2);
+                const bool upper = lane_id() >= 32;
+                treelet_half(upper ? n2 : n1, !upper || k2 != k1, nodes, parents, ws, nf, ts, ci, S);
             }
             if (active && at_node) {
                 cur = node;
