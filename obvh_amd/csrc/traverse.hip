@@ -173,6 +173,29 @@ __device__ __forceinline__ bool enter_unit_box(f3 rd, f3 ro, float hitT, float* 
     return (t0 <= t1) && (t1 > 0.0f) && (t0 <= hitT);
 }
 
+// Fast form of the same slab test: (+-0.5 - ro) * rcp(rd) instead of the
+// correctly rounded quotients.  v_rcp_f32 is within 1 ulp, so with every
+// |rd| in [2^-40, 2^40] and every product in [2^-100, 2^100] (no zero, no
+// overflow, no denormal) each product is within 2^-21 relative of the
+// correctly rounded quotient, and so are the entry / exit distances t0, t1
+// (a max / min of them).  Returns false when those ranges do not hold.
+__device__ __forceinline__ bool slab_fast(f3 rd, f3 ro, float& t0, float& t1) {
+    const float ax = fabsf(rd.x), ay = fabsf(rd.y), az = fabsf(rd.z);
+    const float rx = __builtin_amdgcn_rcpf(rd.x), ry = __builtin_amdgcn_rcpf(rd.y), rz = __builtin_amdgcn_rcpf(rd.z);
+    const float lx = (-0.5f - ro.x) * rx, ly = (-0.5f - ro.y) * ry, lz = (-0.5f - ro.z) * rz;
+    const float hx = (0.5f - ro.x) * rx, hy = (0.5f - ro.y) * ry, hz = (0.5f - ro.z) * rz;
+    const float dmin = fminf(fminf(ax, ay), az), dmax = fmaxf(fmaxf(ax, ay), az);
+    const float pmin = fminf(fminf(fminf(fabsf(lx), fabsf(ly)), fminf(fabsf(lz), fabsf(hx))), fminf(fabsf(hy), fabsf(hz)));
+    const float pmax = fmaxf(fmaxf(fmaxf(fabsf(lx), fabsf(ly)), fmaxf(fabsf(lz), fabsf(hx))), fmaxf(fabsf(hy), fabsf(hz)));
+    t0 = fmaxf(fmaxf(fminf(lx, hx), fminf(ly, hy)), fminf(lz, hz));
+    t1 = fminf(fminf(fmaxf(lx, hx), fmaxf(ly, hy)), fmaxf(lz, hz));
+    return dmin >= 0x1p-40f && dmax <= 0x1p40f && pmin >= 0x1p-100f && pmax <= 0x1p100f;
+}
+
+// x <= y decided from approximations within 2^-21 relative of the exact
+// operands: certain only when they differ by more than 2^-19 (|x| + |y|).
+__device__ __forceinline__ bool sure(float x, float y) { return fabsf(x - y) > 0x1p-19f * (fabsf(x) + fabsf(y)); }
+
 // kr_ray_triangle_intersect (vecmath.h:1656-1676); returns t (t_max when
 // rejected) and the barycentrics of an accepted hit.
 __device__ __forceinline__ float ray_triangle(f3 dir, f3 org, f3 v1, f3 v2, f3 v3, float t_max, float* b1o,
@@ -375,9 +398,29 @@ __global__ __launch_bounds__(kTravThreads) void trace(Tree tree, TraceIO a) {
                 aborted = true;
                 cur = kTravSentinel;
             } else {
-                float tl, tr;
-                const bool go_l = enter_unit_box(m43_dir(L, dir), m43_point(L, org), hitT, &tl);
-                const bool go_r = enter_unit_box(m43_dir(R, dir), m43_point(R, org), hitT, &tr);
+                const f3 rdl = m43_dir(L, dir), rol = m43_point(L, org);
+                const f3 rdr = m43_dir(R, dir), ror = m43_point(R, org);
+                // entry decisions from the fast slab test when each is certain
+                // (t1 > 0 is exact: each product has its quotient's sign),
+                // else from the exact test
+                float tl, tr, t1l, t1r;
+                bool fast = slab_fast(rdl, rol, tl, t1l) && slab_fast(rdr, ror, tr, t1r);
+                bool go_l = false, go_r = false;
+                if (fast) {
+                    if (t1l > 0.0f) {
+                        fast = sure(tl, t1l) && sure(tl, hitT);
+                        go_l = tl <= t1l && tl <= hitT;
+                    }
+                    if (t1r > 0.0f) {
+                        fast = fast && sure(tr, t1r) && sure(tr, hitT);
+                        go_r = tr <= t1r && tr <= hitT;
+                    }
+                    if (go_l && go_r) fast = fast && sure(tr, tl);
+                }
+                if (!fast) {
+                    go_l = enter_unit_box(rdl, rol, hitT, &tl);
+                    go_r = enter_unit_box(rdr, ror, hitT, &tr);
+                }
                 if (go_l != go_r) {
                     if (go_r) next = other;
                 } else if (!go_l) {
