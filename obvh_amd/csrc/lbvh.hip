@@ -86,8 +86,14 @@ __global__ __launch_bounds__(256) void lbvh_centroid_box(const kr_vec3* V, const
         float a = sh[k][0];
         for (int w = 1; w < 4; ++w) a = k < 3 ? fminf(a, sh[k][w]) : fmaxf(a, sh[k][w]);
         if (a == a && a != INF && a != -INF) {
-            if (k < 3) atomicMin(&ws.cbox[k], f2ord(a));
-            else atomicMax(&ws.cbox[k], f2ord(a));
+            // filtered: only a workgroup that improves the current bound
+            // issues the atomic (thousands of same-address atomics serialise)
+            const int32_t key = f2ord(a);
+            const int32_t cur = __hip_atomic_load(&ws.cbox[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (k < 3 ? key < cur : key > cur) {
+                if (k < 3) atomicMin(&ws.cbox[k], key);
+                else atomicMax(&ws.cbox[k], key);
+            }
         }
     }
 }
