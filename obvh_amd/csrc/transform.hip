@@ -3,7 +3,7 @@
 // Method of the reference GPU transform (src/ads/common/cuda/transforms.cu
 // :213-964), restructured for CDNA4:
 //
-//  climb_leaves  one lane per leaf, 1024-leaf workgroups: the leaf OBB and
+//  climb_leaves  one lane per leaf, 512-leaf workgroups: the leaf OBB and
 //                its matrix (transforms.cu:803-834), the leaf's 7-slab k-DOP
 //                (3 vertices, first vertex wins ties), then a bottom-up climb
 //                in which the second child to arrive at a parent merges the
@@ -11,7 +11,7 @@
 //                are counted in LDS; the rest use a 64-bit global arrival
 //                word tagged with the workgroup, and parents whose children
 //                finished in different workgroups are deferred.
-//  climb_phase   the deferred parents, in 32768-leaf workgroups and then one
+//  climb_phase   the deferred parents, in 16384-leaf workgroups and then one
 //                workgroup for the rest (workgroup-scope ordering only; the
 //                kernel boundary publishes everything else).  The merge takes
 //                the right child's extreme only if strictly better, so ties go
@@ -38,8 +38,8 @@
 namespace obvh {
 
 constexpr uint32_t kInvalid = 0xFFFFFFFFu;
-constexpr uint32_t kChunk0 = 1024;    // leaves per workgroup, first climb phase
-constexpr uint32_t kChunk1 = 32768;   // second phase
+constexpr uint32_t kChunk0 = 512;     // leaves per workgroup, first climb phase
+constexpr uint32_t kChunk1 = 16384;   // second phase
 
 struct Mat4 {
     float v[16];
@@ -775,7 +775,7 @@ __global__ __launch_bounds__(256, 4) void candidates(const kr_vec3* __restrict__
 // Exact projection range of every subtree vertex on each internal node's
 // candidate axes (transforms.cu:836-900), as order-preserving int32 keys
 // (-0 orders below +0), min/max over the subtree's vertices.  A workgroup
-// takes a tile of 1024 consecutive leaves and splits each leaf's ancestors
+// takes a tile of 512 consecutive leaves and splits each leaf's ancestors
 // in three tiers:
 //   A  below the lowest ancestor holding all 64 leaves of the lane's wave:
 //      each lane folds its triangle's keys into LDS (a slot per internal
@@ -793,6 +793,7 @@ __global__ __launch_bounds__(256, 4) void candidates(const kr_vec3* __restrict__
 // tile-wide pass (too many distinct vertices, or no tile-common node) tier B
 // continues to the root.
 
+constexpr uint32_t kRTile = 512;   // leaves per refit workgroup
 constexpr int kRHash = 256;      // LDS hash for ancestors outside the tile's index range
 constexpr int kVSet = 2048;      // distinct vertex ids per tile (hash set)
 constexpr int kChainMax = 128;   // ancestors handled by the tile-wide pass per batch
@@ -800,7 +801,7 @@ constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 constexpr int32_t kKeyHi = 0x7FFFFFFF, kKeyLo = (int32_t)0x80000000;
 
 struct RefitTile {
-    int32_t tk[kTile][6];        // keys of internal node lo + s: min b0..b2, max b0..b2
+    int32_t tk[kRTile][6];        // keys of internal node lo + s: min b0..b2, max b0..b2
     uint32_t hkey[kRHash];
     int32_t hval[kRHash][6];
     uint32_t vkey[kVSet];        // distinct vertex ids of the tile's triangles
@@ -832,7 +833,7 @@ __device__ __forceinline__ void fold_lds(int32_t* slot, const int32_t v[6]) {
 }
 
 __device__ __forceinline__ void fold_tile(RefitTile& R, int32_t* range, uint32_t lo, uint32_t p, const int32_t k[6]) {
-    if (p - lo < kTile) {
+    if (p - lo < kRTile) {
         fold_lds(R.tk[p - lo], k);
         return;
     }
@@ -871,15 +872,15 @@ __device__ __forceinline__ bool holds(uint2 sp, uint32_t first, uint32_t last) {
     return sp.x <= first && sp.y >= last && !(sp.y & kGap);
 }
 
-__global__ __launch_bounds__(kTile) void refit(const kr_bvh_node* __restrict__ nodes,
+__global__ __launch_bounds__(kRTile) void refit(const kr_bvh_node* __restrict__ nodes,
                                                const uint32_t* __restrict__ parents,
                                                const kr_vec3* __restrict__ V,
                                                const kr_uvec4* __restrict__ F, TreeWs ws, uint32_t N) {
     __shared__ RefitTile R;
     __shared__ uint32_t bad;
     const uint32_t I = N - 1, t = threadIdx.x;
-    const uint32_t lo = blockIdx.x * kTile;
-    const uint32_t thi = min(lo + kTile, N) - 1;  // last leaf position of the tile
+    const uint32_t lo = blockIdx.x * kRTile;
+    const uint32_t thi = min(lo + kRTile, N) - 1;  // last leaf position of the tile
 #pragma unroll
     for (int a = 0; a < 3; ++a) { R.tk[t][a] = kKeyHi; R.tk[t][3 + a] = kKeyLo; }
     if (t < kRHash) {
@@ -887,7 +888,7 @@ __global__ __launch_bounds__(kTile) void refit(const kr_bvh_node* __restrict__ n
 #pragma unroll
         for (int a = 0; a < 3; ++a) { R.hval[t][a] = kKeyHi; R.hval[t][3 + a] = kKeyLo; }
     }
-    for (uint32_t h = t; h < kVSet; h += kTile) R.vkey[h] = kEmpty;
+    for (uint32_t h = t; h < kVSet; h += kRTile) R.vkey[h] = kEmpty;
     if (t == 0) {
         bad = *ws.error;  // one read per workgroup: the exit is uniform
         R.nchain = 0;
@@ -982,7 +983,7 @@ __global__ __launch_bounds__(kTile) void refit(const kr_bvh_node* __restrict__ n
     __syncthreads();
 
     if (tile_pass) {
-        for (uint32_t h = t; h < kVSet; h += kTile) {
+        for (uint32_t h = t; h < kVSet; h += kRTile) {
             const uint32_t id = R.vkey[h];
             if (id == kEmpty) continue;
             const uint32_t u = atomicAdd(&R.nuniq, 1u);
@@ -994,7 +995,7 @@ __global__ __launch_bounds__(kTile) void refit(const kr_bvh_node* __restrict__ n
         __syncthreads();
         // tier C over the distinct vertices
         const uint32_t nu = R.nuniq, nc = R.nchain;
-        for (uint32_t j = (uint32_t)wave_id(); j < nc; j += kTile / kWave) {
+        for (uint32_t j = (uint32_t)wave_id(); j < nc; j += kRTile / kWave) {
             const uint32_t q = __builtin_amdgcn_readfirstlane(R.chain[j]);
             const Cand c = ws.cand[q];
             if (c.degen) continue;
@@ -1162,7 +1163,7 @@ kr_error tree_launch(const kr_bvh_node* nodes, const uint32_t* parents, const kr
     auto blocks = [](uint32_t n, uint32_t b) { return dim3((n + b - 1) / b); };
     hipLaunchKernelGGL(tree_clear, dim3(1024), dim3(256), 0, stream, ws.visit, I + 1, ws);
     hipLaunchKernelGGL(leaf_boxes, blocks(N, kStage), dim3(kStage), 0, stream, nodes, V, F, obbs, mats, N);
-    // climb: 1024-leaf tiles, then 32768-leaf workgroups, then one workgroup
+    // climb: 512-leaf tiles, then 16384-leaf workgroups, then one workgroup
     // for the rest
     hipLaunchKernelGGL(climb_leaves, blocks(N, kTile), dim3(kTile), 0, stream, nodes, parents, V, F, ws, N);
     if (N > kTile) {
@@ -1177,7 +1178,7 @@ kr_error tree_launch(const kr_bvh_node* nodes, const uint32_t* parents, const kr
     }
     if (I > 0) {
         hipLaunchKernelGGL(candidates, blocks(I, 256), dim3(256), 0, stream, V, ws, I);
-        hipLaunchKernelGGL(refit, blocks(N, kTile), dim3(kTile), 0, stream, nodes, parents, V, F, ws, N);
+        hipLaunchKernelGGL(refit, blocks(N, kRTile), dim3(kRTile), 0, stream, nodes, parents, V, F, ws, N);
         hipLaunchKernelGGL(finalize, blocks(I, kStage), dim3(kStage), 0, stream, ws, obbs, mats, I);
     }
     if (ids)
