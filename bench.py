@@ -13,6 +13,9 @@ roofline = algorithmic bytes of one transform call
 cpu_baseline = the CPU oracle (a single-thread C port of the same method) on
            the same tree, timed on this host; its output is also compared
            bit-exactly with the GPU output ("parity").
+cpu_reference_transform = the reference's own CPU transform (kr_bvh_obb_tree,
+           a full DiTO of every subtree; a different method) restated in C,
+           single thread, on the subtrees of the first 262144 leaves.
 Further legs (reported in the same JSON line): collapse + SAH of the tree,
 OBVH packing and closest-hit traversal of 1M camera and 1M random rays
 through the collapsed OBB tree, 2M-triangle transforms (LBVH and ATRBVH
@@ -400,6 +403,20 @@ def transform_leg(V, F, label: str, steps: int, warmup: int, with_cpu: bool, bui
             "sample": f"whole workload: one transform of the {N}-triangle tree ({cpu_s:.2f} s, single thread)",
         }
         print(f"[bench] cpu oracle {label}: {cpu_s:.2f}s, parity={out['parity']}", file=sys.stderr, flush=True)
+        # the reference's own CPU transform (kr_bvh_obb_tree: a full DiTO of
+        # every subtree), a different method, timed on the subtrees of the
+        # first 262144 leaves
+        le = min(N, 262144)
+        tc = time.perf_counter()
+        _, _, _, fitted = oracle.cpu_obb_tree(nodes_h, parents_h, V, F, 0, le)
+        ref_s = time.perf_counter() - tc
+        out["cpu_reference_transform"] = {
+            "value": round(fitted / ref_s / 1e6, 4), "unit": "Mnodes/s", "cores": 1, "kind": "port",
+            "sample": f"kr_bvh_obb_tree restated: the {fitted} internal nodes below the first {le} leaves "
+                      f"({ref_s:.2f} s, single thread)",
+        }
+        print(f"[bench] reference CPU transform {label}: {fitted} nodes in {ref_s:.2f}s", file=sys.stderr,
+              flush=True)
     if trace:
         try:
             out["traversal"] = traversal_leg(V, F, Vd, Fd, col["collapsed"], mats, col["primitives"], label,
@@ -514,6 +531,7 @@ def main() -> int:
             "device_ms": round(dev_ms_mean, 5),
         },
         "cpu_baseline": cpu,
+        "cpu_reference_transform": main_leg.get("cpu_reference_transform"),
         "parity": parity,
         "collapse_ms": main_leg["collapse_ms"],
         "sah": main_leg["sah"],
