@@ -432,8 +432,8 @@ __global__ __launch_bounds__(1024) void atr_treelets(kr_bvh_node* nodes, uint32_
     const uint32_t tag = (phase << 24) | (blockIdx.x + 1u);
     const uint64_t lo = (uint64_t)blockIdx.x * chunk, hi = lo + chunk;
     const uint32_t nin = phase == 0 ? 0 : min(ws.ph.nlist[phase - 1], I);
-    const uint32_t* in = phase == 0 ? nullptr : ws.ph.list + (size_t)(phase - 1) * (I + 1);
-    uint32_t* out = ws.ph.list + (size_t)phase * (I + 1);
+    const uint2* in = phase == 0 ? nullptr : ws.ph.list + (size_t)(phase - 1) * (I + 1);
+    uint2* out = ws.ph.list + (size_t)phase * (I + 1);
     const uint32_t nstart = phase == 0 ? chunk : nin;
     for (uint32_t base = 0; base < nstart; base += blockDim.x) {  // uniform over the workgroup
         const uint32_t r0 = base + threadIdx.x;
@@ -453,19 +453,22 @@ __global__ __launch_bounds__(1024) void atr_treelets(kr_bvh_node* nodes, uint32_
                     active = true;
                 }
             } else {
-                cur = in[r0];
-                if (cur < I) {
+                // Ownership comes from the listed key only: once its owner has
+                // completed the node and an ancestor, a treelet there may
+                // rewire the node's children while other workgroups still
+                // scan this list.
+                const uint2 e = in[r0];
+                cur = e.x;
+                if (cur < I && e.y >= lo && e.y < hi) {
                     const uint32_t l = nodes[cur].u.children.left, r = nodes[cur].u.children.right;
                     const uint2 sa = ws.ph.span[l], sb = ws.ph.span[r];
                     span = make_uint2(min(sa.x, sb.x), max(sa.y, sb.y));
-                    if (span.x >= lo && span.x < hi) {
-                        if (span.y >= hi) {
-                            const uint32_t slot = atomicAdd(&ws.ph.nlist[phase], 1u);
-                            if (slot < I) out[slot] = cur;
-                        } else {
-                            active = true;
-                            at_node = true;  // both children are complete
-                        }
+                    if (span.x < lo || span.y >= hi) {
+                        const uint32_t slot = atomicAdd(&ws.ph.nlist[phase], 1u);
+                        if (slot < I) out[slot] = make_uint2(cur, span.x);
+                    } else {
+                        active = true;
+                        at_node = true;  // both children are complete
                     }
                 }
             }
@@ -484,7 +487,7 @@ __global__ __launch_bounds__(1024) void atr_treelets(kr_bvh_node* nodes, uint32_
                         active = false;  // first arrival: the sibling continues
                     } else if (t1 != tag && (t1 >> 24) >= phase) {
                         const uint32_t slot = atomicAdd(&ws.ph.nlist[phase], 1u);
-                        if (slot < I) out[slot] = p;
+                        if (slot < I) out[slot] = make_uint2(p, span.x);
                         active = false;
                     } else {
                         const uint32_t l = nodes[p].u.children.left, r = nodes[p].u.children.right;

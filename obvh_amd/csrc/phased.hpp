@@ -5,7 +5,13 @@
 // Phase k runs workgroups that own leaf positions [w*chunk_k, (w+1)*chunk_k)
 // (chunks 1024, 32768, then one workgroup for everything).  Phase 0 starts
 // one lane per leaf; later phases start from the nodes the previous phase
-// deferred, taken by the workgroup that owns their first leaf.  A lane that
+// deferred.  Each deferred node is listed with a key, a leaf position of its
+// subtree known to the lane that deferred it, and is taken by the workgroup
+// whose chunk holds the key; that workgroup runs it when the node's whole
+// leaf span lies in its chunk and defers it again otherwise.  Other
+// workgroups decide from the key alone and never read the node: a pass that
+// restructures the tree (ATRBVH treelets) may rewire a deferred node's
+// children as soon as its owner has completed it and an ancestor.  A lane that
 // completes a child adds (tag << 32) | 1 to the parent's 64-bit arrival word
 // (tag = (phase << 24) | (workgroup + 1)); the first arrival stops, the
 // second continues if the first came from an earlier kernel (visible after
@@ -26,14 +32,14 @@ constexpr int kPhases = 3;  // chunk 1024, chunk 32768, everything
 struct PhaseWs {
     uint64_t* visit;   // per internal node: arrivals (low word) + sum of tags (high word)
     uint32_t* nlist;   // per phase: number of deferred nodes
-    uint32_t* list;    // per phase: deferred nodes (I + 1 entries each)
+    uint2* list;       // per phase: deferred nodes and their keys (I + 1 entries each)
     uint2* span;       // per node: first and last leaf position below it
 };
 
 inline void phase_carve(PhaseWs* ws, Carver& c, uint32_t nf) {
     ws->visit = c.take<uint64_t>((size_t)nf + 1);
     ws->nlist = c.take<uint32_t>(4);
-    ws->list = c.take<uint32_t>((size_t)kPhases * ((size_t)nf + 1));
+    ws->list = c.take<uint2>((size_t)kPhases * ((size_t)nf + 1));
     ws->span = c.take<uint2>(2 * (size_t)nf);
 }
 
@@ -56,8 +62,8 @@ __device__ __forceinline__ void phased_up(const kr_bvh_node* nodes, const uint32
     const uint32_t tag = (phase << 24) | (blockIdx.x + 1u);
     const uint64_t lo = (uint64_t)blockIdx.x * chunk, hi = lo + chunk;
     const uint32_t nin = phase == 0 ? 0 : min(ws.nlist[phase - 1], I);
-    const uint32_t* in = phase == 0 ? nullptr : ws.list + (size_t)(phase - 1) * (I + 1);
-    uint32_t* out = ws.list + (size_t)phase * (I + 1);
+    const uint2* in = phase == 0 ? nullptr : ws.list + (size_t)(phase - 1) * (I + 1);
+    uint2* out = ws.list + (size_t)phase * (I + 1);
     const uint32_t nstart = phase == 0 ? chunk : nin;
     for (uint32_t r0 = threadIdx.x; r0 < nstart; r0 += blockDim.x) {
         uint32_t cur;
@@ -69,15 +75,15 @@ __device__ __forceinline__ void phased_up(const kr_bvh_node* nodes, const uint32
             op.leaf(cur);
             span = make_uint2((uint32_t)pos, (uint32_t)pos);
         } else {
-            cur = in[r0];
-            if (cur >= I) continue;
+            const uint2 e = in[r0];
+            cur = e.x;
+            if (cur >= I || e.y < lo || e.y >= hi) continue;  // another workgroup owns it
             const uint32_t l = nodes[cur].u.children.left, r = nodes[cur].u.children.right;
             const uint2 sa = ws.span[l], sb = ws.span[r];
             span = make_uint2(min(sa.x, sb.x), max(sa.y, sb.y));
-            if (span.x < lo || span.x >= hi) continue;  // another workgroup owns its first leaf
-            if (span.y >= hi) {                         // wider than this workgroup: next phase
+            if (span.x < lo || span.y >= hi) {  // wider than this workgroup: next phase
                 const uint32_t slot = atomicAdd(&ws.nlist[phase], 1u);
-                if (slot < I) out[slot] = cur;
+                if (slot < I) out[slot] = make_uint2(cur, span.x);
                 continue;
             }
             op.merge(cur, l, r);
@@ -93,7 +99,7 @@ __device__ __forceinline__ void phased_up(const kr_bvh_node* nodes, const uint32
             const uint32_t t1 = (uint32_t)(old >> 32);
             if (t1 != tag && (t1 >> 24) >= phase) {  // sibling done by another workgroup of this phase
                 const uint32_t slot = atomicAdd(&ws.nlist[phase], 1u);
-                if (slot < I) out[slot] = p;
+                if (slot < I) out[slot] = make_uint2(p, span.x);
                 break;
             }
             const uint32_t sib = (l == cur) ? r : l;
