@@ -10,9 +10,11 @@ value = internal nodes / wall time per step (Mnodes/s), inputs resident in HBM.
 roofline = algorithmic bytes of one transform call
            (2N-1)(36 node + 4 parent + 60 obb + 64 mat4) + 16 N + 12 V
            over its device time (hipEvents on the launch stream).
-cpu_baseline = the CPU oracle (a single-thread C port of the same method) on
-           the same tree, timed on this host; its output is also compared
-           bit-exactly with the GPU output ("parity").
+cpu_baseline = the CPU oracle (a C port of the same method) on the same tree,
+           node-parallel on the host's cores (at most 16 threads), timed on
+           this host; cpu_baseline_1core = the same oracle on one thread.
+           Both outputs are compared bit-exactly with the GPU output
+           ("parity").
 cpu_reference_transform = the reference's own CPU transform (kr_bvh_obb_tree,
            a full DiTO of every subtree; a different method) restated in C,
            single thread, on the subtrees of the first 262144 leaves.
@@ -173,9 +175,25 @@ def dito_legs(args) -> dict:
             pts += b - a
             if t_cpu > 8.0:
                 break
-        out["cpu_baseline"] = {"Gpts_per_s": round(pts / t_cpu / 1e9, 4), "sets_per_s": round(done / t_cpu, 1),
-                               "cores": 1, "kind": "port",
-                               "sample": f"first {done} sets of the batch ({pts} points, {t_cpu:.1f} s)"}
+        out["cpu_baseline_1core"] = {"Gpts_per_s": round(pts / t_cpu / 1e9, 4),
+                                     "sets_per_s": round(done / t_cpu, 1), "cores": 1, "kind": "port",
+                                     "sample": f"first {done} sets of the batch ({pts} points, {t_cpu:.1f} s)"}
+        # all host cores, sets in parallel, on the whole batch; compared with
+        # the GPU's boxes set by set
+        thr = oracle.host_threads()
+        P = np.stack([xh, yh, zh], 1)
+        del xh, yh, zh
+        tc = time.perf_counter()
+        exp, _ = oracle.points_obb_batch(P, offs, threads=thr)
+        t_all = time.perf_counter() - tc
+        del P
+        out["parity"] = bool(np.array_equal(exp, batched.cpu().numpy()))
+        out["cpu_baseline"] = {"Gpts_per_s": round(total / t_all / 1e9, 4), "sets_per_s": round(args.sets / t_all, 1),
+                               "cores": thr, "kind": "port",
+                               "sample": f"whole batch: {args.sets} sets ({total} points, {t_all:.2f} s, "
+                                         f"{thr} threads, sets in parallel)"}
+        print(f"[bench] cpu DiTO: whole batch in {t_all:.2f}s on {thr} threads, parity={out['parity']}",
+              file=sys.stderr, flush=True)
     return out
 
 
@@ -397,12 +415,24 @@ def transform_leg(V, F, label: str, steps: int, warmup: int, with_cpu: bool, bui
         tc = time.perf_counter()
         eo, em = oracle.obb_tree(nodes_h, parents_h, V, F)
         cpu_s = time.perf_counter() - tc
-        out["parity"] = bool(np.array_equal(obbs.cpu().numpy(), eo) and np.array_equal(mats.cpu().numpy(), em))
+        thr = oracle.host_threads()
+        tc = time.perf_counter()
+        eo_mt, em_mt = oracle.obb_tree(nodes_h, parents_h, V, F, threads=thr)
+        cpu_mt = time.perf_counter() - tc
+        go, gm = obbs.cpu().numpy(), mats.cpu().numpy()
+        out["parity"] = bool(np.array_equal(go, eo) and np.array_equal(gm, em)
+                             and np.array_equal(go, eo_mt) and np.array_equal(gm, em_mt))
         out["cpu_baseline"] = {
+            "value": round(internal / cpu_mt / 1e6, 4), "unit": "Mnodes/s", "cores": thr, "kind": "port",
+            "sample": f"whole workload: one transform of the {N}-triangle tree ({cpu_mt:.2f} s, {thr} threads, "
+                      f"node-parallel)",
+        }
+        out["cpu_baseline_1core"] = {
             "value": round(internal / cpu_s / 1e6, 4), "unit": "Mnodes/s", "cores": 1, "kind": "port",
             "sample": f"whole workload: one transform of the {N}-triangle tree ({cpu_s:.2f} s, single thread)",
         }
-        print(f"[bench] cpu oracle {label}: {cpu_s:.2f}s, parity={out['parity']}", file=sys.stderr, flush=True)
+        print(f"[bench] cpu oracle {label}: {cpu_s:.2f}s single thread, {cpu_mt:.2f}s on {thr} threads, "
+              f"parity={out['parity']}", file=sys.stderr, flush=True)
         # the reference's own CPU transform (kr_bvh_obb_tree: a full DiTO of
         # every subtree), a different method, timed on the subtrees of the
         # first 262144 leaves
@@ -442,7 +472,7 @@ def main() -> int:
     ap.add_argument("--no-trace", action="store_true", help="skip the OBVH packing + traversal leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run two 1 GiB device copies (calibration of the HBM byte counters)")
-    ap.add_argument("--serve-seconds", type=float, default=10.0, help="open-loop serving duration")
+    ap.add_argument("--serve-seconds", type=float, default=60.0, help="open-loop serving duration (s)")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="open-loop arrival rate (req/s)")
     args = ap.parse_args()
     if args.copy_probe:

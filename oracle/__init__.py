@@ -36,8 +36,11 @@ def lib():
         L.oracle_dito14.argtypes = [vp, C.c_int, vp, vp]
         L.oracle_points_obb.argtypes = [vp, C.c_int, vp, vp]
         L.oracle_points_obb_batch.argtypes = [vp, vp, C.c_int, vp, vp]
+        L.oracle_points_obb_batch_mt.argtypes = [vp, vp, C.c_int, vp, vp, C.c_int]
         L.oracle_obb_tree.argtypes = [vp, vp, vp, vp, C.c_uint32, vp, vp, vp, vp, vp]
         L.oracle_obb_tree.restype = C.c_int
+        L.oracle_obb_tree_mt.argtypes = [vp, vp, vp, vp, C.c_uint32, vp, vp, vp, vp, vp, C.c_int]
+        L.oracle_obb_tree_mt.restype = C.c_int
         L.oracle_lbvh_build.argtypes = [vp, vp, C.c_uint32, vp, vp, vp, vp, vp]
         L.oracle_lbvh_build.restype = C.c_int
         L.oracle_cpu_obb_tree.argtypes = [vp, vp, vp, vp, C.c_uint32, vp, vp, C.c_uint32, C.c_uint32]
@@ -87,13 +90,15 @@ def points_obb(points, trace: bool = False):
     return (out, tr[0]) if trace else out
 
 
-def points_obb_batch(points, offsets):
+def points_obb_batch(points, offsets, threads: int = 1):
+    """kr_points_obb of every set s = points[offsets[s]:offsets[s+1]];
+    threads > 1 fits sets in parallel on host threads (same outputs)."""
     p = _f32(points).reshape(-1, 3)
     off = np.ascontiguousarray(offsets, dtype=np.uint64)
     n = off.size - 1
     out = np.zeros((n, 15), dtype=np.float32)
     tr = np.zeros(n, dtype=TRACE_DTYPE)
-    lib().oracle_points_obb_batch(_p(p), _p(off), n, _p(out), _p(tr))
+    lib().oracle_points_obb_batch_mt(_p(p), _p(off), n, _p(out), _p(tr), int(threads))
     return out, tr
 
 
@@ -113,10 +118,21 @@ def lbvh_build(vertices, faces):
     return nodes, parents, counts, costs, mortons
 
 
+def host_threads() -> int:
+    """Host threads for an all-cores oracle run: the cores this process may
+    use, at most 16 (a one-GPU share of a GPU host)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
 def obb_tree(nodes, parents, vertices, faces, with_ids: bool = False, with_ranges: bool = False,
-             with_axes: bool = False):
+             with_axes: bool = False, threads: int = 1):
     """The transform method.  Returns obbs (2N-1, 15), mats (2N-1, 16)
-    [, ids (2N-1, 14)] [, ranges (N-1, 6)] [, candidate axes (N-1, 10)]."""
+    [, ids (2N-1, 14)] [, ranges (N-1, 6)] [, candidate axes (N-1, 10)].
+    threads > 1 runs it node-parallel on host threads (same outputs)."""
     V = _f32(vertices).reshape(-1, 3)
     F = np.ascontiguousarray(faces, dtype=np.uint32).reshape(-1, 4)
     nodes = np.ascontiguousarray(nodes, dtype=NODE_DTYPE)
@@ -128,10 +144,10 @@ def obb_tree(nodes, parents, vertices, faces, with_ids: bool = False, with_range
     ids = np.zeros((total, 14), dtype=np.uint32) if with_ids else None
     rr = np.zeros((max(N - 1, 1), 6), dtype=np.float32) if with_ranges else None
     ca = np.zeros((max(N - 1, 1), 10), dtype=np.float32) if with_axes else None
-    rc = lib().oracle_obb_tree(
+    rc = lib().oracle_obb_tree_mt(
         _p(nodes), _p(parents), _p(V), _p(F), N, _p(obbs), _p(mats),
         _p(ids) if ids is not None else None, _p(rr) if rr is not None else None,
-        _p(ca) if ca is not None else None)
+        _p(ca) if ca is not None else None, int(threads))
     if rc != 0:
         raise ValueError("oracle_obb_tree: malformed tree")
     res = [obbs, mats]

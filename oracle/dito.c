@@ -311,8 +311,26 @@ void oracle_points_obb(const float* xyz, int n, oobb* out, odito_trace* trace) {
 
 void oracle_points_obb_batch(const float* xyz, const uint64_t* off, int nsets,
                              oobb* out, odito_trace* traces) {
-    for (int s = 0; s < nsets; ++s) {
-        oracle_points_obb(xyz + 3 * off[s], (int)(off[s + 1] - off[s]), &out[s],
-                          traces ? &traces[s] : 0);
-    }
+    oracle_points_obb_batch_mt(xyz, off, nsets, out, traces, 1);
+}
+
+typedef struct {
+    const float* xyz;
+    const uint64_t* off;
+    oobb* out;
+    odito_trace* traces;
+} batch_ctx;
+
+static void batch_one(void* p, uint32_t s, int tid) {
+    const batch_ctx* c = (const batch_ctx*)p;
+    (void)tid;
+    oracle_points_obb(c->xyz + 3 * c->off[s], (int)(c->off[s + 1] - c->off[s]), &c->out[s],
+                      c->traces ? &c->traces[s] : 0);
+}
+
+void oracle_points_obb_batch_mt(const float* xyz, const uint64_t* off, int nsets,
+                                oobb* out, odito_trace* traces, int threads) {
+    if (nsets <= 0) return;
+    batch_ctx c = {xyz, off, out, traces};
+    oracle_parallel_for((uint32_t)nsets, 1, threads, batch_one, &c);
 }

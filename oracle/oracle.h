@@ -63,6 +63,17 @@ void oracle_points_obb(const float* xyz, int n, oobb* out, odito_trace* trace);
 /* Many sets back to back: set s = points [off[s], off[s+1]). */
 void oracle_points_obb_batch(const float* xyz, const uint64_t* off, int nsets,
                              oobb* out, odito_trace* traces);
+/* The same on `threads` host threads (sets in parallel, same results). */
+void oracle_points_obb_batch_mt(const float* xyz, const uint64_t* off, int nsets,
+                                oobb* out, odito_trace* traces, int threads);
+
+/* Parallel for (par.c): fn(ctx, i, thread id) for i in [0, n), chunks of
+ * `chunk` indices handed out dynamically to `threads` POSIX threads
+ * (thread ids 0..threads-1; 1 or fewer runs inline).  Returns the number
+ * of threads that ran. */
+#define OPAR_MAX_THREADS 256
+typedef void (*opar_fn)(void* ctx, uint32_t i, int tid);
+int oracle_parallel_for(uint32_t n, uint32_t chunk, int threads, opar_fn fn, void* ctx);
 
 /* The GPU transform method.  nodes/parents: 2N-1 entries (N = nprim);
  * outputs obbs/mats for every node.  kdop_ids (nullable): 14 ids per node
@@ -75,6 +86,12 @@ int oracle_obb_tree(const onode* nodes, const uint32_t* parents,
                     const float* vertices, const uint32_t* faces /* uvec4 */,
                     uint32_t nprim, oobb* obbs, omat4* mats,
                     uint32_t* kdop_ids, float* refit_ranges, float* cand_axes);
+/* The same on `threads` host threads: candidates, refit + finalize and leaf
+ * boxes run node-parallel; outputs are identical for every thread count. */
+int oracle_obb_tree_mt(const onode* nodes, const uint32_t* parents,
+                       const float* vertices, const uint32_t* faces,
+                       uint32_t nprim, oobb* obbs, omat4* mats,
+                       uint32_t* kdop_ids, float* refit_ranges, float* cand_axes, int threads);
 
 /* LBVH build.  vertices AoS xyz, faces uvec4.  Outputs 2N-1 nodes, parents,
  * primitive_counts, costs; mortons_sorted (N entries, nullable). */

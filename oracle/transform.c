@@ -1,7 +1,7 @@
 /*
  * transform.c -- CPU restatement of the reference's parallel AABB-BVH ->
  * OBB-tree transform (src/ads/common/cuda/transforms.cu:213-964), run
- * sequentially.
+ * sequentially or node-parallel on host threads (same results).
  *
  * TEST INFRASTRUCTURE ONLY (the checker for the HIP transform).
  *
@@ -364,93 +364,144 @@ static int check_tree(const onode* nodes, const uint32_t* parents, uint32_t N) {
     return 0;
 }
 
+typedef struct {
+    const onode* nodes;
+    const float* V;
+    const uint32_t* faces;
+    uint32_t I;
+    kdop* K;
+    candidate* C;
+    oobb* obbs;
+    omat4* mats;
+    float* refit_ranges;
+    float* cand_axes;
+    uint32_t* stacks; /* one DFS stack of `stack_len` entries per thread */
+    uint32_t stack_len;
+} tree_ctx;
+
+static void candidates_one(void* p, uint32_t v, int tid) {
+    const tree_ctx* t = (const tree_ctx*)p;
+    (void)tid;
+    candidate_search(t->V, &t->K[v], &t->C[v]);
+}
+
+/* Refit: exact ranges over every subtree vertex (transforms.cu:836-900),
+ * computed per node over its own subtree; then the node's box (K4). */
+static void refit_one(void* p, uint32_t v, int tid) {
+    const tree_ctx* t = (const tree_ctx*)p;
+    const onode* nodes = t->nodes;
+    const uint32_t I = t->I;
+    const candidate* c = &t->C[v];
+    uint32_t* stack = t->stacks + (size_t)tid * t->stack_len;
+    ov3 rmin = c->rmin, rmax = c->rmax;
+    if (!c->degenerate) {
+        uint32_t sp = 0;
+        stack[sp++] = v;
+        while (sp) {
+            uint32_t u = stack[--sp];
+            if (u < I) { stack[sp++] = nodes[u].a; stack[sp++] = nodes[u].b; continue; }
+            const uint32_t* f = &t->faces[4 * nodes[u].a];
+            for (int j = 0; j < 3; ++j) {
+                ov3 q = vert(t->V, f[j]);
+                float d0 = vdot(q, c->b0), d1 = vdot(q, c->b1), d2 = vdot(q, c->b2);
+                rmin.x = kmin(rmin.x, d0); rmax.x = kmax(rmax.x, d0);
+                rmin.y = kmin(rmin.y, d1); rmax.y = kmax(rmax.y, d1);
+                rmin.z = kmin(rmin.z, d2); rmax.z = kmax(rmax.z, d2);
+            }
+        }
+    }
+    if (t->cand_axes) {
+        float* ca = &t->cand_axes[10 * v];
+        ca[0] = c->b0.x; ca[1] = c->b0.y; ca[2] = c->b0.z;
+        ca[3] = c->b1.x; ca[4] = c->b1.y; ca[5] = c->b1.z;
+        ca[6] = c->b2.x; ca[7] = c->b2.y; ca[8] = c->b2.z;
+        ca[9] = (float)c->degenerate;
+    }
+    if (t->refit_ranges) {
+        float* rr = &t->refit_ranges[6 * v];
+        rr[0] = rmin.x; rr[1] = rmin.y; rr[2] = rmin.z;
+        rr[3] = rmax.x; rr[4] = rmax.y; rr[5] = rmax.z;
+    }
+    finalize_node(&t->K[v], c, rmin, rmax, &t->obbs[v], &t->mats[v]);
+}
+
+static void leaf_one(void* p, uint32_t i, int tid) {
+    const tree_ctx* t = (const tree_ctx*)p;
+    (void)tid;
+    const uint32_t v = t->I + i;
+    const uint32_t* f = &t->faces[4 * t->nodes[v].a];
+    leaf_obb(vert(t->V, f[0]), vert(t->V, f[1]), vert(t->V, f[2]), &t->obbs[v], &t->mats[v]);
+}
+
 int oracle_obb_tree(const onode* nodes, const uint32_t* parents, const float* V,
                     const uint32_t* faces, uint32_t N, oobb* obbs, omat4* mats,
                     uint32_t* kdop_ids, float* refit_ranges, float* cand_axes) {
+    return oracle_obb_tree_mt(nodes, parents, V, faces, N, obbs, mats, kdop_ids, refit_ranges, cand_axes, 1);
+}
+
+int oracle_obb_tree_mt(const onode* nodes, const uint32_t* parents, const float* V,
+                       const uint32_t* faces, uint32_t N, oobb* obbs, omat4* mats,
+                       uint32_t* kdop_ids, float* refit_ranges, float* cand_axes, int threads) {
     if (N == 0) return 0;
     uint32_t I = N - 1, total = 2 * N - 1;
     if (check_tree(nodes, parents, N) != 0) return -1;
+    if (threads < 1) threads = 1;
+    if (threads > OPAR_MAX_THREADS) threads = OPAR_MAX_THREADS;
 
     kdop* K = (kdop*)malloc(sizeof(kdop) * total);
     candidate* C = (candidate*)malloc(sizeof(candidate) * (I ? I : 1));
     uint32_t* order = (uint32_t*)malloc(sizeof(uint32_t) * total);
-    if (!K || !C || !order) { free(K); free(C); free(order); return -1; }
+    uint32_t* depth = (uint32_t*)malloc(sizeof(uint32_t) * total);
+    if (!K || !C || !order || !depth) { free(K); free(C); free(order); free(depth); return -1; }
 
-    /* Post-order of the tree (children before parents), iterative DFS. */
+    /* Post-order of the tree (children before parents), iterative DFS; the
+     * deepest node bounds the refit's DFS stacks. */
+    uint32_t max_depth = 0;
     {
         uint32_t* stack = (uint32_t*)malloc(sizeof(uint32_t) * (total + 1));
         uint8_t* seen = (uint8_t*)calloc(total, 1);
         uint32_t sp = 0, cnt = 0;
         stack[sp++] = 0;
+        depth[0] = 0;
         while (sp) {
             uint32_t v = stack[sp - 1];
             if (v >= I || seen[v]) { order[cnt++] = v; --sp; continue; }
             seen[v] = 1;
+            const uint32_t d = depth[v] + 1;
+            if (d > max_depth) max_depth = d;
+            depth[nodes[v].a] = d;
+            depth[nodes[v].b] = d;
             stack[sp++] = nodes[v].b;
             stack[sp++] = nodes[v].a;
         }
         free(stack); free(seen);
-        if (cnt != total) { free(K); free(C); free(order); return -1; }
+        if (cnt != total) { free(K); free(C); free(order); free(depth); return -1; }
     }
+    free(depth);
 
+    /* K1 in post-order (sequential: each merge needs its children). */
     for (uint32_t t = 0; t < total; ++t) {
         uint32_t v = order[t];
-        if (v >= I) {
-            leaf_kdop(V, &faces[4 * nodes[v].a], &K[v]);
-        } else {
-            merge_kdop(&K[nodes[v].a], &K[nodes[v].b], &K[v]);
-            candidate_search(V, &K[v], &C[v]);
-        }
+        if (v >= I) leaf_kdop(V, &faces[4 * nodes[v].a], &K[v]);
+        else merge_kdop(&K[nodes[v].a], &K[nodes[v].b], &K[v]);
     }
+    free(order);
 
-    /* Refit: exact ranges over every subtree vertex (transforms.cu:836-900),
-     * computed per node over its own subtree. */
-    uint32_t* stack = (uint32_t*)malloc(sizeof(uint32_t) * (total + 1));
-    for (uint32_t v = 0; v < I; ++v) {
-        const candidate* c = &C[v];
-        ov3 rmin = c->rmin, rmax = c->rmax;
-        if (!c->degenerate) {
-            uint32_t sp = 0;
-            stack[sp++] = v;
-            while (sp) {
-                uint32_t u = stack[--sp];
-                if (u < I) { stack[sp++] = nodes[u].a; stack[sp++] = nodes[u].b; continue; }
-                const uint32_t* f = &faces[4 * nodes[u].a];
-                for (int j = 0; j < 3; ++j) {
-                    ov3 p = vert(V, f[j]);
-                    float d0 = vdot(p, c->b0), d1 = vdot(p, c->b1), d2 = vdot(p, c->b2);
-                    rmin.x = kmin(rmin.x, d0); rmax.x = kmax(rmax.x, d0);
-                    rmin.y = kmin(rmin.y, d1); rmax.y = kmax(rmax.y, d1);
-                    rmin.z = kmin(rmin.z, d2); rmax.z = kmax(rmax.z, d2);
-                }
-            }
-        }
-        if (cand_axes) {
-            float* ca = &cand_axes[10 * v];
-            ca[0] = c->b0.x; ca[1] = c->b0.y; ca[2] = c->b0.z;
-            ca[3] = c->b1.x; ca[4] = c->b1.y; ca[5] = c->b1.z;
-            ca[6] = c->b2.x; ca[7] = c->b2.y; ca[8] = c->b2.z;
-            ca[9] = (float)c->degenerate;
-        }
-        if (refit_ranges) {
-            float* rr = &refit_ranges[6 * v];
-            rr[0] = rmin.x; rr[1] = rmin.y; rr[2] = rmin.z;
-            rr[3] = rmax.x; rr[4] = rmax.y; rr[5] = rmax.z;
-        }
-        finalize_node(&K[v], c, rmin, rmax, &obbs[v], &mats[v]);
-    }
-    free(stack);
+    /* A DFS from a node at depth d holds at most (max_depth - d) + 2 entries. */
+    tree_ctx ctx = {nodes, V, faces, I, K, C, obbs, mats, refit_ranges, cand_axes, NULL, max_depth + 2};
+    ctx.stacks = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)ctx.stack_len * (size_t)threads);
+    if (!ctx.stacks) { free(K); free(C); return -1; }
+    oracle_parallel_for(I, 64, threads, candidates_one, &ctx);  /* K2 */
+    oracle_parallel_for(I, 16, threads, refit_one, &ctx);       /* K3 ranges + K4 */
+    oracle_parallel_for(N, 1024, threads, leaf_one, &ctx);      /* K3 leaf boxes */
+    free(ctx.stacks);
 
-    for (uint32_t v = I; v < total; ++v) {
-        const uint32_t* f = &faces[4 * nodes[v].a];
-        leaf_obb(vert(V, f[0]), vert(V, f[1]), vert(V, f[2]), &obbs[v], &mats[v]);
-    }
     if (kdop_ids) {
         for (uint32_t v = 0; v < total; ++v) {
             memcpy(&kdop_ids[14 * v], K[v].imn, 7 * sizeof(uint32_t));
             memcpy(&kdop_ids[14 * v + 7], K[v].imx, 7 * sizeof(uint32_t));
         }
     }
-    free(K); free(C); free(order);
+    free(K); free(C);
     return 0;
 }

@@ -153,3 +153,14 @@ def test_batch_equals_single(oracle_lib):
     for s in range(sizes.size):
         one = oracle_lib.points_obb(pts[off[s]:off[s + 1]])
         assert np.array_equal(out[s], one)
+
+
+@pytest.mark.parametrize("threads", [2, 3, 8])
+def test_threaded_batch_equals_sequential(oracle_lib, threads):
+    """The all-cores timing run fits the same boxes, traces included."""
+    sizes = np.array([0, 1, 3, 14, 15, 100, 5000] * 9 + [20000, 7])
+    pts, off = synth.batch_sets(sizes, seed=4)
+    out1, tr1 = oracle_lib.points_obb_batch(pts, off)
+    outn, trn = oracle_lib.points_obb_batch(pts, off, threads=threads)
+    assert np.array_equal(out1.view(np.uint32), outn.view(np.uint32))
+    assert tr1.tobytes() == trn.tobytes()

@@ -243,6 +243,19 @@ def test_planar_mesh_gives_aabbs(oracle_lib):
     assert np.array_equal(ext[:, :2], np.maximum(half[:, :2], np.float32(0.001)))
 
 
+@pytest.mark.parametrize("threads", [2, 5, 8])
+def test_threaded_transform_equals_sequential(oracle_lib, threads):
+    """The all-cores timing run gives the same boxes, matrices, extremal ids,
+    ranges and axes as the sequential oracle."""
+    V, F = synth.hairball(tubes=400, seed=5)
+    nodes, parents, _, _, _ = oracle_lib.lbvh_build(V, F)
+    seq = oracle_lib.obb_tree(nodes, parents, V, F, with_ids=True, with_ranges=True, with_axes=True)
+    par = oracle_lib.obb_tree(nodes, parents, V, F, with_ids=True, with_ranges=True, with_axes=True,
+                              threads=threads)
+    for a, b in zip(seq, par):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
 def test_malformed_tree_is_rejected(oracle_lib, tree):
     nodes = tree["nodes"].copy()
     parents = tree["parents"].copy()
