@@ -208,6 +208,8 @@ __global__ void tree_clear(uint64_t* visit, uint32_t n, TreeWs ws) {
     if (t == 0) {
         *ws.error = 0;
         for (int k = 0; k < kMaxPhases; ++k) ws.nlist[k] = 0;
+        *ws.wide_count = 0;
+        *ws.wide_gap = 0;
     }
     for (uint32_t i = t; i < n; i += gridDim.x * blockDim.x) visit[i] = 0;
 }
@@ -510,29 +512,46 @@ __global__ __launch_bounds__(1024) void climb_phase(const kr_bvh_node* __restric
         }
         kdop_store(ws.kdop, cur, k);
         ws.span[cur] = span;
+        // One memory round trip per level: the next level's topology (read
+        // only) is fetched a level ahead, and the sibling's k-DOP and span
+        // are read together with the arrival atomic.  That early read is
+        // used only when the sibling was completed by an earlier kernel
+        // (visible since the launch); a sibling completed by this workgroup
+        // in this phase is read again after the acquire.
         uint32_t p = parents[cur];
+        uint32_t l = 0, rr = 0, pp = kInvalid;
+        if (p < I) { l = nodes[p].u.children.left; rr = nodes[p].u.children.right; pp = parents[p]; }
         for (uint32_t step = 0; p != kInvalid; ++step) {
             if (p >= I || step >= N) { atomicOr(ws.error, 1u); break; }
-            const uint32_t l = nodes[p].u.children.left, rr = nodes[p].u.children.right;
-            const uint32_t pp = parents[p];
+            const bool am_left = (l == cur);
+            const uint32_t sib = am_left ? rr : l;
+            if (sib >= total || (!am_left && rr != cur)) { atomicOr(ws.error, 2u); break; }
+            Kdop o;
+            kdop_load(ws.kdop, sib, o);
+            uint2 so = span_of(sib);
+            uint32_t nl = 0, nr = 0, npp = kInvalid;
+            if (pp < I) { nl = nodes[pp].u.children.left; nr = nodes[pp].u.children.right; npp = parents[pp]; }
             const uint64_t old = __hip_atomic_fetch_add(&ws.visit[p], ((uint64_t)tag << 32) | 1ull,
                                                         __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
             const uint32_t cnt = (uint32_t)old, t1 = (uint32_t)(old >> 32);
             if (cnt == 0) break;  // first arrival: the sibling continues
             if (cnt != 1) { atomicOr(ws.error, 2u); break; }
             if (t1 != tag && (t1 >> 24) >= phase) { defer(ws, phase, p, I); break; }
-            const bool am_left = (l == cur);
-            const uint32_t sib = am_left ? rr : l;
-            if (sib >= total || (!am_left && rr != cur)) { atomicOr(ws.error, 2u); break; }
-            Kdop o, m;
-            kdop_load(ws.kdop, sib, o);
+            if (t1 == tag) {  // completed by this workgroup in this kernel
+                kdop_load(ws.kdop, sib, o);
+                so = span_of(sib);
+            }
+            Kdop m;
             if (am_left) merge_left_first(k, o, m); else merge_left_first(o, k, m);
             k = m;
-            span = span_join(span, span_of(sib));
+            span = span_join(span, so);
             kdop_store(ws.kdop, p, k);
             ws.span[p] = span;
             cur = p;
             p = pp;
+            l = nl;
+            rr = nr;
+            pp = npp;
         }
     }
 }
@@ -770,6 +789,54 @@ __global__ __launch_bounds__(256, 4) void candidates(const kr_vec3* __restrict__
     rk[3] = f2ord(b.dmax.x); rk[4] = f2ord(b.dmax.y); rk[5] = f2ord(b.dmax.z);
 }
 
+// Wide nodes for the refit's tile-wide pass.  Workgroup w lists internal
+// nodes [w * chunk, (w + 1) * chunk): it counts its wide nodes, takes their
+// slots with one global atomic, then writes them (slots from an LDS
+// counter).  A few hundred workgroups keep the same-address atomics few.
+constexpr uint32_t kWideBlocks = 256;
+__device__ __forceinline__ bool wide_node(uint2 sp, bool& gap_wide) {
+    const bool gap = (sp.y & kGap) != 0;
+    const uint32_t hi = span_hi(sp), first = (sp.x + kRTile - 1) / kRTile * kRTile;
+    gap_wide = gap && hi - sp.x >= kRTile - 1;
+    return !gap && first <= hi && hi - first >= kRTile - 1;
+}
+
+__global__ __launch_bounds__(1024) void list_wide(TreeWs ws, uint32_t I) {
+    __shared__ uint32_t cnt, gap, base, off;
+    const uint32_t t = threadIdx.x;
+    const uint32_t chunk = (I + gridDim.x - 1) / gridDim.x;
+    const uint32_t b0 = blockIdx.x * chunk, b1 = min(b0 + chunk, I);
+    if (t == 0) { cnt = 0; gap = 0; off = 0; }
+    __syncthreads();
+    uint32_t mine = 0;
+    bool g = false;
+    for (uint32_t node = b0 + t; node < b1; node += blockDim.x) {
+        bool gw;
+        mine += wide_node(ws.span[node], gw) ? 1u : 0u;
+        g |= gw;
+    }
+    if (mine != 0) atomicAdd(&cnt, mine);  // LDS
+    if (g) gap = 1;
+    __syncthreads();
+    if (t == 0) {
+        if (cnt != 0) base = atomicAdd(ws.wide_count, cnt);
+        if (gap != 0 && __hip_atomic_load(ws.wide_gap, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+            atomicOr(ws.wide_gap, 1u);
+    }
+    __syncthreads();
+    if (cnt == 0) return;  // uniform
+    for (uint32_t node = b0 + t; node < b1; node += blockDim.x) {
+        const uint2 sp = ws.span[node];
+        bool gw;
+        if (!wide_node(sp, gw)) continue;
+        const uint32_t slot = base + atomicAdd(&off, 1u);
+        if (slot < kWideMax) {
+            ws.wide_span[slot] = sp;
+            ws.wide_node[slot] = node;
+        }
+    }
+}
+
 // ------------------------------------------------------------ refit --
 //
 // Exact projection range of every subtree vertex on each internal node's
@@ -792,23 +859,58 @@ __global__ __launch_bounds__(256, 4) void candidates(const kr_vec3* __restrict__
 // folded into global memory with filtered atomics at the end.  Without the
 // tile-wide pass (too many distinct vertices, or no tile-common node) tier B
 // continues to the root.
+// The walks of tiers A and B read the parent, span and axes of nodes in the
+// tile's own index range [lo, lo + 512) from an LDS copy made at the start
+// (a Karras tree numbers every node below the tile-common node inside that
+// range; other nodes are read from global memory).  The ancestors for tier C
+// are the wide nodes whose span holds the tile, found by one parallel scan of
+// the wide-node list instead of a walk up the parent links (which would be
+// ~30 dependent loads deep); when the list cannot describe them (a spanning
+// node with non-contiguous leaves, a list overflow, the short last tile) wave
+// 0 walks the parent links.
 
-constexpr uint32_t kRTile = 512;   // leaves per refit workgroup
 constexpr int kRHash = 256;      // LDS hash for ancestors outside the tile's index range
 constexpr int kVSet = 2048;      // distinct vertex ids per tile (hash set)
 constexpr int kChainMax = 128;   // ancestors handled by the tile-wide pass per batch
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 constexpr int32_t kKeyHi = 0x7FFFFFFF, kKeyLo = (int32_t)0x80000000;
 
+struct RefitNodes {              // internal nodes lo .. lo + 511 (tiers A and B)
+    float b[9][kRTile];          // candidate axes
+    uint2 span[kRTile];
+    uint32_t parent[kRTile];
+    uint8_t degen[kRTile];
+};
+
 struct RefitTile {
     int32_t tk[kRTile][6];        // keys of internal node lo + s: min b0..b2, max b0..b2
     uint32_t hkey[kRHash];
     int32_t hval[kRHash][6];
     uint32_t vkey[kVSet];        // distinct vertex ids of the tile's triangles
-    float upos[kVSet][3];        // their positions, compacted
+    union {
+        RefitNodes nd;           // until tiers A and B end
+        float upos[kVSet][3];    // then: the distinct vertices' positions, compacted
+    };
     uint32_t chain[kChainMax];   // tile-common node and its ancestors
     uint32_t nchain, nuniq, overflow, rest;
 };
+
+// Parent, span and candidate axes of internal node p for tiers A and B.
+__device__ __forceinline__ void refit_node(const RefitTile& R, const TreeWs& ws, const uint32_t* parents,
+                                           uint32_t lo, uint32_t p, uint32_t& parent, uint2& span, Cand& c) {
+    const uint32_t s = p - lo;
+    if (s < kRTile) {
+        parent = R.nd.parent[s];
+        span = R.nd.span[s];
+#pragma unroll
+        for (int j = 0; j < 9; ++j) c.b[j] = R.nd.b[j][s];
+        c.degen = R.nd.degen[s];
+    } else {
+        parent = parents[p];
+        span = ws.span[p];
+        c = ws.cand[p];
+    }
+}
 
 // Order-preserving key of a float (same map as f2ord, without a branch).
 __device__ __forceinline__ int32_t okey(float f) {
@@ -877,7 +979,7 @@ __global__ __launch_bounds__(kRTile) void refit(const kr_bvh_node* __restrict__ 
                                                const kr_vec3* __restrict__ V,
                                                const kr_uvec4* __restrict__ F, TreeWs ws, uint32_t N) {
     __shared__ RefitTile R;
-    __shared__ uint32_t bad;
+    __shared__ uint32_t bad, listed;
     const uint32_t I = N - 1, t = threadIdx.x;
     const uint32_t lo = blockIdx.x * kRTile;
     const uint32_t thi = min(lo + kRTile, N) - 1;  // last leaf position of the tile
@@ -889,8 +991,21 @@ __global__ __launch_bounds__(kRTile) void refit(const kr_bvh_node* __restrict__ 
         for (int a = 0; a < 3; ++a) { R.hval[t][a] = kKeyHi; R.hval[t][3 + a] = kKeyLo; }
     }
     for (uint32_t h = t; h < kVSet; h += kRTile) R.vkey[h] = kEmpty;
+    if (lo + t < I) {  // the tile's own index range, for tiers A and B
+        const uint32_t node = lo + t;
+        const Cand c = ws.cand[node];
+#pragma unroll
+        for (int j = 0; j < 9; ++j) R.nd.b[j][t] = c.b[j];
+        R.nd.degen[t] = (uint8_t)(c.degen != 0);
+        R.nd.span[t] = ws.span[node];
+        R.nd.parent[t] = parents[node];
+    }
     if (t == 0) {
         bad = *ws.error;  // one read per workgroup: the exit is uniform
+        // the wide-node list describes this tile's ancestors: a whole tile,
+        // no spanning node with a gap, every wide node listed
+        const uint32_t wc = *ws.wide_count;
+        listed = (thi - lo + 1 == kRTile && *ws.wide_gap == 0 && wc <= kWideMax) ? wc + 1u : 0u;
         R.nchain = 0;
         R.nuniq = 0;
         R.overflow = 0;
@@ -898,6 +1013,16 @@ __global__ __launch_bounds__(kRTile) void refit(const kr_bvh_node* __restrict__ 
     }
     __syncthreads();
     if (bad != 0) return;
+    const uint32_t nwide = listed;  // wide nodes + 1, or 0: walk the parent links
+    if (nwide != 0) {
+        for (uint32_t i = t; i + 1 < nwide; i += kRTile) {
+            const uint2 sp = ws.wide_span[i];
+            if (sp.x <= lo && sp.y >= thi) {
+                const uint32_t slot = atomicAdd(&R.nchain, 1u);
+                if (slot < (uint32_t)kChainMax) R.chain[slot] = ws.wide_node[i];
+            }
+        }
+    }
 
     const uint32_t pos = lo + t;
     const uint32_t wlo = pos - (uint32_t)lane_id();
@@ -925,6 +1050,10 @@ __global__ __launch_bounds__(kRTile) void refit(const kr_bvh_node* __restrict__ 
     }
     __syncthreads();
     const bool tile_pass = R.overflow == 0;
+    // the list held more ancestors than one batch: walk the parent links
+    const bool use_list = nwide != 0 && R.nchain <= (uint32_t)kChainMax;
+    __syncthreads();  // every lane has read nchain before wave 0 may reset it
+    if (!use_list && t == 0) R.nchain = 0;
 
     if (wlo < N) {  // wave-uniform
         // tier A
@@ -932,14 +1061,17 @@ __global__ __launch_bounds__(kRTile) void refit(const kr_bvh_node* __restrict__ 
         if (valid) {
             uint32_t p = parents[I + pos];
             for (uint32_t step = 0; p != kInvalid && p < I && step < N; ++step) {
-                if (holds(ws.span[p], wlo, whi)) { common = p; break; }
-                const Cand c = ws.cand[p];
+                uint32_t pp;
+                uint2 sp;
+                Cand c;
+                refit_node(R, ws, parents, lo, p, pp, sp, c);
+                if (holds(sp, wlo, whi)) { common = p; break; }
                 if (!c.degen) {
                     int32_t k[6];
                     tri_keys(c, v0, v1, v2, k);
                     fold_tile(R, ws.range, lo, p, k);
                 }
-                p = parents[p];
+                p = pp;
             }
         }
         // tier B; lane 0 is valid (wlo < N) and a well-formed tree gives
@@ -950,8 +1082,11 @@ __global__ __launch_bounds__(kRTile) void refit(const kr_bvh_node* __restrict__ 
         } else {
             uint32_t q = c0;
             for (uint32_t step = 0; q != kInvalid && q < I && step < N; ++step) {  // uniform
-                if (tile_pass && holds(ws.span[q], lo, thi)) break;  // tier C takes q and above
-                const Cand c = ws.cand[q];
+                uint32_t qp;
+                uint2 sp;
+                Cand c;
+                refit_node(R, ws, parents, lo, q, qp, sp, c);
+                if (tile_pass && holds(sp, lo, thi)) break;  // tier C takes q and above
                 if (!c.degen) {
                     int32_t k[6];
                     if (valid) {
@@ -963,10 +1098,11 @@ __global__ __launch_bounds__(kRTile) void refit(const kr_bvh_node* __restrict__ 
                     wave_keys(k);
                     if (lane_id() == 0) fold_tile(R, ws.range, lo, q, k);
                 }
-                q = __builtin_amdgcn_readfirstlane(parents[q]);
+                q = __builtin_amdgcn_readfirstlane(qp);
             }
-            // wave 0 stopped at the tile-common node: record it and its ancestors
-            if (tile_pass && t < kWave && q != kInvalid && q < I) {
+            // wave 0 stopped at the tile-common node: without the list,
+            // record it and its ancestors
+            if (!use_list && tile_pass && t < kWave && q != kInvalid && q < I) {
                 uint32_t n = 0;
                 while (q != kInvalid && q < I && n < (uint32_t)kChainMax) {
                     if (lane_id() == 0) R.chain[n] = q;
@@ -1039,13 +1175,14 @@ __global__ __launch_bounds__(kRTile) void refit(const kr_bvh_node* __restrict__ 
     // fold the tile's LDS keys into global memory.  A node whose leaves all
     // lie in this tile has no contributor outside this workgroup, and within
     // it only the LDS slot (tier C, which folds with atomics, never takes
-    // such a node except the tile-common node): its keys are merged with a
-    // plain read-modify-write.  The others are merged with atomics.
+    // such a node except the tile-common node, whose span is the tile): its
+    // keys are merged with a plain read-modify-write.  The others are merged
+    // with atomics.
     if (lo + t < I && (R.tk[t][0] != kKeyHi || R.tk[t][3] != kKeyLo)) {
         const uint32_t node = lo + t;
         const uint2 sp = ws.span[node];
         int32_t* rk = &ws.range[(size_t)node * 6];
-        const bool chain_node = R.nchain > 0 && R.chain[0] == node;
+        const bool chain_node = tile_pass && holds(sp, lo, thi);
         if (sp.x >= lo && span_hi(sp) <= thi && !chain_node) {
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
@@ -1143,12 +1280,16 @@ size_t tree_carve(TreeWs* ws, void* base, size_t cap, uint32_t N) {
     const size_t I = N ? N - 1 : 0, total = N ? 2 * (size_t)N - 1 : 0;
     ws->visit = c.take<uint64_t>(I + 1);
     ws->error = c.take<uint32_t>(8);
-    ws->nlist = ws->error ? ws->error + 2 : nullptr;
+    ws->nlist = ws->error ? ws->error + 2 : nullptr;  // error[2 .. 2 + kMaxPhases)
+    ws->wide_count = ws->error ? ws->error + 6 : nullptr;
+    ws->wide_gap = ws->error ? ws->error + 7 : nullptr;
     ws->list = c.take<uint32_t>((size_t)kMaxPhases * (I + 1));
     ws->span = c.take<uint2>(total);
     ws->kdop = c.take<float4>(total * 7);
     ws->cand = c.take<Cand>(I + 1);
     ws->range = c.take<int32_t>((I + 1) * 6);
+    ws->wide_span = c.take<uint2>(kWideMax);
+    ws->wide_node = c.take<uint32_t>(kWideMax);
     return c.used;
 }
 
@@ -1177,6 +1318,7 @@ kr_error tree_launch(const kr_bvh_node* nodes, const uint32_t* parents, const kr
         }
     }
     if (I > 0) {
+        hipLaunchKernelGGL(list_wide, dim3(min(kWideBlocks, (I + 1023) / 1024)), dim3(1024), 0, stream, ws, I);
         hipLaunchKernelGGL(candidates, blocks(I, 256), dim3(256), 0, stream, V, ws, I);
         hipLaunchKernelGGL(refit, blocks(N, kRTile), dim3(kRTile), 0, stream, nodes, parents, V, F, ws, N);
         hipLaunchKernelGGL(finalize, blocks(I, kStage), dim3(kStage), 0, stream, ws, obbs, mats, I);

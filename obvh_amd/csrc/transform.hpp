@@ -25,7 +25,20 @@ struct TreeWs {
     float4* kdop;      // 7 per node: {min, min vertex id, max, max vertex id}
     Cand* cand;        // per internal node
     int32_t* range;    // 6 per internal node: order-preserving keys, min b0..b2, max b0..b2
+    // Wide nodes: internal nodes whose contiguous leaf span holds at least
+    // one whole aligned refit tile, listed by the candidates kernel (span and
+    // node id; at most kWideMax).  wide_count counts every such node (it may
+    // exceed the list), wide_gap is nonzero when a node spanning a tile or
+    // more has non-contiguous leaves (the list then does not describe the
+    // ancestors and the refit walks parent links instead).
+    uint2* wide_span;
+    uint32_t* wide_node;
+    uint32_t* wide_count;
+    uint32_t* wide_gap;
 };
+constexpr uint32_t kWideMax = 16384;
+constexpr uint32_t kRTile = 512;  // leaves per refit workgroup (tiles aligned to multiples of it)
+static_assert(2 + kMaxPhases <= 6, "error words 6 and 7 hold the wide-node count and gap flag");
 
 size_t tree_carve(TreeWs* ws, void* base, size_t cap, uint32_t N);
 kr_error tree_launch(const kr_bvh_node* nodes, const uint32_t* parents, const kr_vec3* V,

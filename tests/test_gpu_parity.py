@@ -13,6 +13,7 @@ pytestmark = pytest.mark.gpu
 
 from obvh_amd import ops, synth  # noqa: E402
 from obvh_amd.layouts import NODE_DTYPE, TRACE_DTYPE  # noqa: E402
+from treeutil import contiguous_tree  # noqa: E402
 
 
 @pytest.fixture(scope="module")
@@ -245,6 +246,30 @@ def test_obb_tree_random_topology(torch, oracle_lib, n):
     V, F = synth.hairball(tubes=n // 40 + 1, seed=6)
     F = F[:n]
     nodes, parents = random_tree(n, seed=n)
+    Vd = torch.from_numpy(V).cuda()
+    Fd = torch.from_numpy(np.ascontiguousarray(F).view(np.int32)).cuda()
+    nd = torch.from_numpy(nodes.view(np.uint8)).cuda()
+    pd = torch.from_numpy(parents.view(np.int32)).cuda()
+    obbs, mats, ids = ops.obb_tree(nd, pd, Vd, Fd, with_ids=True)
+    torch.cuda.synchronize()
+    eo, em, ei = oracle_lib.obb_tree(nodes, parents, V, F, with_ids=True)
+    assert np.array_equal(ids.cpu().numpy().view(np.uint32), ei)
+    assert np.array_equal(obbs.cpu().numpy(), eo)
+    assert np.array_equal(mats.cpu().numpy(), em)
+
+
+@pytest.mark.parametrize("n,spine", [(30000, 0), (81920, 512), (81920, 384)])
+def test_obb_tree_contiguous_topology(torch, oracle_lib, n, spine):
+    """Trees whose subtrees hold contiguous leaf runs but whose internal nodes
+    are numbered breadth-first (not Karras): the refit finds each tile's
+    ancestors in the wide-node list while its LDS copy of the tile's index
+    range mostly misses.  spine=512: the leftmost tile has 127 ancestors
+    holding it, within one batch of 128; spine=384: 171, more than one
+    batch, so those tiles walk the parent links instead."""
+    say(f"[obb_tree] contiguous topology n={n} spine={spine}")
+    V, F = synth.hairball(tubes=n // 40 + 1, seed=7)
+    F = F[:n]
+    nodes, parents = contiguous_tree(n, seed=n, spine_block=spine)
     Vd = torch.from_numpy(V).cuda()
     Fd = torch.from_numpy(np.ascontiguousarray(F).view(np.int32)).cuda()
     nd = torch.from_numpy(nodes.view(np.uint8)).cuda()

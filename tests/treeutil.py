@@ -42,6 +42,45 @@ def random_tree(n_leaves: int, seed: int):
     return nodes, parents
 
 
+def contiguous_tree(n_leaves: int, seed: int, spine_block: int = 0):
+    """Random binary tree in which every subtree holds a contiguous run of
+    leaf positions (leaf node I + f holds face f), internal nodes numbered
+    breadth-first rather than as a Karras tree.  spine_block > 0: a node with
+    more than 2 * spine_block leaves peels a block of spine_block to
+    1.5 * spine_block leaves off its right end, so the leftmost leaves lie
+    below a chain of about n_leaves / spine_block ancestors."""
+    rng = np.random.default_rng(seed)
+    I = n_leaves - 1
+    nodes = np.zeros(2 * n_leaves - 1, dtype=NODE_DTYPE)
+    parents = np.zeros(2 * n_leaves - 1, dtype=np.uint32)
+    parents[0] = INV
+    queue = [(0, 0, n_leaves)]  # (node, first leaf, leaf count)
+    next_internal = 1
+    head = 0
+    while head < len(queue):
+        idx, first, cnt = queue[head]
+        head += 1
+        if spine_block and cnt > 2 * spine_block:
+            cut = cnt - spine_block - int(rng.integers(0, spine_block // 2 + 1))
+        else:
+            cut = int(rng.integers(1, cnt))
+        kids = []
+        for f0, c in ((first, cut), (first + cut, cnt - cut)):
+            if c == 1:
+                kids.append(I + f0)
+            else:
+                kids.append(next_internal)
+                queue.append((next_internal, f0, c))
+                next_internal += 1
+        nodes[idx]["a"], nodes[idx]["b"] = kids
+        for k in kids:
+            parents[k] = idx
+    leaf = nodes[I:]
+    leaf["a"] = np.arange(n_leaves)
+    leaf["nprim"] = 1
+    return nodes, parents
+
+
 def fill_bounds(nodes, V, F):
     """Leaf AABBs from their triangles, internal AABBs as unions (in place)."""
     total = nodes.shape[0]
