@@ -1129,26 +1129,59 @@ __global__ __launch_bounds__(kRTile) void refit(const kr_bvh_node* __restrict__ 
             R.upos[u][2] = p.z;
         }
         __syncthreads();
-        // tier C over the distinct vertices
+        // tier C over the distinct vertices; each wave reads the axes and
+        // current keys of its next ancestor while it works on this one (the
+        // keys only filter the atomics: they only move outwards, so an early
+        // read never skips a needed update)
         const uint32_t nu = R.nuniq, nc = R.nchain;
-        for (uint32_t j = (uint32_t)wave_id(); j < nc; j += kRTile / kWave) {
-            const uint32_t q = __builtin_amdgcn_readfirstlane(R.chain[j]);
-            const Cand c = ws.cand[q];
-            if (c.degen) continue;
-            int32_t k[6];
+        const uint32_t step_c = kRTile / kWave;
+        uint32_t j = (uint32_t)wave_id();
+        uint32_t qc = 0;
+        Cand c = {};
+        int32_t cur[6] = {0, 0, 0, 0, 0, 0};
+        if (j < nc) {
+            qc = __builtin_amdgcn_readfirstlane(R.chain[j]);
+            c = ws.cand[qc];
 #pragma unroll
-            for (int a = 0; a < 3; ++a) { k[a] = kKeyHi; k[3 + a] = kKeyLo; }
-            for (uint32_t u = (uint32_t)lane_id(); u < nu; u += kWave) {
-                const f3 p = f3{R.upos[u][0], R.upos[u][1], R.upos[u][2]};
+            for (int a = 0; a < 6; ++a) cur[a] = ws.range[(size_t)qc * 6 + a];
+        }
+        for (; j < nc; j += step_c) {
+            uint32_t qn = 0;
+            Cand cn = {};
+            int32_t curn[6] = {0, 0, 0, 0, 0, 0};
+            if (j + step_c < nc) {
+                qn = __builtin_amdgcn_readfirstlane(R.chain[j + step_c]);
+                cn = ws.cand[qn];
 #pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    const int32_t kk = okey(dot3(p, f3{c.b[3 * a], c.b[3 * a + 1], c.b[3 * a + 2]}));
-                    k[a] = min(k[a], kk);
-                    k[3 + a] = max(k[3 + a], kk);
+                for (int a = 0; a < 6; ++a) curn[a] = ws.range[(size_t)qn * 6 + a];
+            }
+            if (!c.degen) {
+                int32_t k[6];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) { k[a] = kKeyHi; k[3 + a] = kKeyLo; }
+                for (uint32_t u = (uint32_t)lane_id(); u < nu; u += kWave) {
+                    const f3 p = f3{R.upos[u][0], R.upos[u][1], R.upos[u][2]};
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        const int32_t kk = okey(dot3(p, f3{c.b[3 * a], c.b[3 * a + 1], c.b[3 * a + 2]}));
+                        k[a] = min(k[a], kk);
+                        k[3 + a] = max(k[3 + a], kk);
+                    }
+                }
+                wave_keys(k);
+                if (lane_id() == 0) {
+                    int32_t* rk = &ws.range[(size_t)qc * 6];
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        if (k[a] < cur[a]) atomicMin(&rk[a], k[a]);
+                        if (k[3 + a] > cur[3 + a]) atomicMax(&rk[3 + a], k[3 + a]);
+                    }
                 }
             }
-            wave_keys(k);
-            if (lane_id() == 0) fold_global(&ws.range[(size_t)q * 6], k);
+            qc = qn;
+            c = cn;
+#pragma unroll
+            for (int a = 0; a < 6; ++a) cur[a] = curn[a];
         }
         // ancestors beyond the recorded batch (very deep trees): every wave
         // walks them with its lanes' triangles
