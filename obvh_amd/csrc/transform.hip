@@ -32,6 +32,8 @@
 // the decisions documented in the oracle (oracle/transform.c).
 #include <float.h>
 
+#include <mutex>
+
 #include "common.hpp"
 #include "transform.hpp"
 
@@ -1326,6 +1328,34 @@ size_t tree_carve(TreeWs* ws, void* base, size_t cap, uint32_t N) {
     return c.used;
 }
 
+// Per device: a second stream (default priority) and the events that fork
+// leaf_boxes onto it and join it back.  One lock covers a whole call's
+// enqueue sequence, so calls from several host threads cannot interleave
+// their record / wait pairs.
+namespace {
+struct SideStream {
+    hipStream_t s = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+};
+constexpr int kMaxDevices = 64;
+std::mutex g_side_mu;
+SideStream g_side[kMaxDevices];
+
+kr_error side_stream(SideStream** out) {
+    int dev = 0;
+    OBVH_HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return "obvh_obb_tree: device ordinal out of range";
+    SideStream& ss = g_side[dev];
+    if (ss.s == nullptr) {
+        OBVH_HIP_TRY(hipStreamCreateWithFlags(&ss.s, hipStreamNonBlocking));
+        OBVH_HIP_TRY(hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming));
+        OBVH_HIP_TRY(hipEventCreateWithFlags(&ss.join, hipEventDisableTiming));
+    }
+    *out = &ss;
+    return nullptr;
+}
+}  // namespace
+
 kr_error tree_launch(const kr_bvh_node* nodes, const uint32_t* parents, const kr_vec3* V,
                      const kr_uvec4* F, kr_obb3* obbs, kr_mat4* mats, uint32_t* ids, uint32_t N,
                      void* workspace, size_t ws_bytes, hipStream_t stream) {
@@ -1335,11 +1365,26 @@ kr_error tree_launch(const kr_bvh_node* nodes, const uint32_t* parents, const kr
     if (workspace == nullptr || ws_bytes < need) return "obvh_obb_tree: workspace too small";
     const uint32_t I = N - 1;
     auto blocks = [](uint32_t n, uint32_t b) { return dim3((n + b - 1) / b); };
+    std::lock_guard<std::mutex> lock(g_side_mu);
+    SideStream* side = nullptr;
+    if (kr_error e = side_stream(&side)) return e;
     hipLaunchKernelGGL(tree_clear, dim3(1024), dim3(256), 0, stream, ws.visit, I + 1, ws);
-    hipLaunchKernelGGL(leaf_boxes, blocks(N, kStage), dim3(kStage), 0, stream, nodes, V, F, obbs, mats, N);
     // climb: 512-leaf tiles, then 16384-leaf workgroups, then one workgroup
     // for the rest
     hipLaunchKernelGGL(climb_leaves, blocks(N, kTile), dim3(kTile), 0, stream, nodes, parents, V, F, ws, N);
+    // The leaf boxes need none of the other kernels: they run on the side
+    // stream beside the 16384-leaf climb phase (60-odd workgroups), which
+    // they slow by about half their own time.  Beside the last phase (one
+    // workgroup) or finalize they saved nothing: the fork's event adds a gap
+    // of ~7 us to the launch stream and the co-running kernel stretches.
+    auto fork_leaf_boxes = [&]() -> kr_error {
+        OBVH_HIP_TRY(hipEventRecord(side->fork, stream));
+        OBVH_HIP_TRY(hipStreamWaitEvent(side->s, side->fork, 0));
+        hipLaunchKernelGGL(leaf_boxes, blocks(N, kStage), dim3(kStage), 0, side->s, nodes, V, F, obbs, mats, N);
+        OBVH_HIP_TRY(hipEventRecord(side->join, side->s));
+        return nullptr;
+    };
+    if (kr_error e = fork_leaf_boxes()) return e;
     if (N > kTile) {
         uint32_t phase = 1;
         for (uint32_t chunk : {kChunk1, 0u}) {
@@ -1358,6 +1403,7 @@ kr_error tree_launch(const kr_bvh_node* nodes, const uint32_t* parents, const kr
     }
     if (ids)
         hipLaunchKernelGGL(copy_ids, blocks(2 * N - 1, 256), dim3(256), 0, stream, nodes, V, F, ws, ids, 2 * N - 1);
+    OBVH_HIP_TRY(hipStreamWaitEvent(stream, side->join, 0));
     OBVH_HIP_TRY(hipGetLastError());
     return nullptr;
 }
