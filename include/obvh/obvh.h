@@ -179,8 +179,9 @@ kr_error obvh_dito_soa(const float* x, const float* y, const float* z, kr_size c
 
 /* Batched DiTO over `nsets` sets stored back to back in SoA arrays;
  * set s is points [offsets[s], offsets[s+1]).  offsets: device, nsets+1
- * entries.  tile_points: points per work tile (multiple of 256; 0 = auto).
- * One persistent launch per pass; the whole call is graph-capturable.     */
+ * entries.  tile_points: points per work tile (multiple of 512; 0 = auto).
+ * workers: workgroups per pass, each looping over tiles (0 = one per tile,
+ * at most 65536).  One launch per pass; the whole call is graph-capturable. */
 size_t obvh_dito_batch_workspace_size(u32 nsets, kr_size total_points, u32 tile_points);
 kr_error obvh_dito_batch(const float* x, const float* y, const float* z,
                          const kr_size* offsets, u32 nsets, kr_size total_points,

@@ -9,7 +9,8 @@
 //  setup     one workgroup scans the set sizes into a tile table (a set is
 //            cut into tiles of `tile` points) and writes the result of empty
 //            sets; a grid kernel fills tile -> set.
-//  pass 1    persistent workgroups (4 x wave64) stream tiles.  Lane l of a
+//  pass 1    workgroups (4 x wave64) take tiles in grid-stride order (one
+//            tile per workgroup by default).  Lane l of a
 //            wave reads points chunk + j*64 + l, j < 8 (coalesced SoA or AoS),
 //            forms the 7 slab projections, and keeps per slab side the running
 //            extreme with the index where it was first reached.  The 8-point
@@ -645,7 +646,9 @@ kr_error dito_launch(const float* x, const float* y, const float* z, const float
     const uint64_t* offs = offsets ? offsets : ws.offsets_single;
     obvh_dito_trace* tr = trace ? trace : ws.trace_tmp;
     const size_t T = dito_max_tiles(nsets, total, tile);
-    if (workers == 0) workers = (uint32_t)(T < 4096 ? T : 4096);
+    // default: one workgroup per tile (measured faster than persistent
+    // workgroups looping over tiles)
+    if (workers == 0) workers = (uint32_t)(T < 65536 ? T : 65536);
     if (workers > T) workers = (uint32_t)T;
     hipLaunchKernelGGL(dito_setup, dim3(1), dim3(1024), 0, stream, offsets, single_count, nsets, tile, ws, out, tr);
     uint32_t tb = (uint32_t)((T + 255) / 256);
