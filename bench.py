@@ -320,8 +320,19 @@ def traversal_leg(V, F, Vd, Fd, nodes, mats, prims, label: str, steps: int, warm
             rec["nodes_per_ray"] = round(float(st[:, 0].mean()), 2)
             rec["triangles_per_ray"] = round(float(st[:, 1].mean()), 2)
             rec["max_stack"] = int(st[:, 2].max())
-            rec["cpu_baseline"] = {"value": round(sample.shape[0] / cpu_s / 1e6, 4), "unit": "Mrays/s", "cores": 1,
-                                   "kind": "port", "sample": f"every 16th ray ({sample.shape[0]} rays, {cpu_s:.2f} s)"}
+            rec["cpu_baseline_1core"] = {"value": round(sample.shape[0] / cpu_s / 1e6, 4), "unit": "Mrays/s",
+                                         "cores": 1, "kind": "port",
+                                         "sample": f"every 16th ray ({sample.shape[0]} rays, {cpu_s:.2f} s)"}
+            # all host cores on the whole batch, compared ray by ray with the GPU
+            thr = oracle.host_threads()
+            tc = time.perf_counter()
+            exp_all = oracle.obvh_intersect(L, R, M, V, F, ph, rays, threads=thr)
+            cpu_all = time.perf_counter() - tc
+            rec["parity"] = rec["parity"] and bool(
+                np.array_equal(np.ascontiguousarray(got).view(np.uint8), np.ascontiguousarray(exp_all).view(np.uint8)))
+            rec["cpu_baseline"] = {"value": round(n / cpu_all / 1e6, 4), "unit": "Mrays/s", "cores": thr,
+                                   "kind": "port",
+                                   "sample": f"whole batch ({n} rays, {cpu_all:.2f} s, {thr} threads, rays in parallel)"}
         print(f"[bench] trace {label} {kind}: {n} rays, records {dev_r:.4f} ms device "
               f"({rec['Mrays_per_s']} Mrays/s), SoA {dev:.4f} ms ({rec['soa']['Mrays_per_s']} Mrays/s), "
               f"records == SoA {rec['records_equal_soa']}, hits {rec['hit_fraction']}, "

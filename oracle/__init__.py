@@ -56,6 +56,8 @@ def lib():
         L.oracle_obvh_pack.restype = C.c_int
         L.oracle_obvh_intersect.argtypes = [vp, vp, vp, vp, vp, vp, vp, C.c_uint32, vp, vp]
         L.oracle_obvh_intersect.restype = C.c_int
+        L.oracle_obvh_intersect_mt.argtypes = [vp, vp, vp, vp, vp, vp, vp, C.c_uint32, vp, vp, C.c_int]
+        L.oracle_obvh_intersect_mt.restype = C.c_int
         L.oracle_ray_unit_box.argtypes = [vp, vp, vp]
         L.oracle_ray_triangle.argtypes = [vp, vp, vp, vp, vp, C.c_float, vp]
         L.oracle_minverse4.argtypes = [vp, vp]
@@ -251,10 +253,11 @@ def obvh_pack(nodes, mats):
     return L, R, meta
 
 
-def obvh_intersect(L, R, meta, vertices, faces, primitives, rays, with_stats: bool = False):
+def obvh_intersect(L, R, meta, vertices, faces, primitives, rays, with_stats: bool = False, threads: int = 1):
     """soa_obvh_persistent_intersect for every ray (RAY_DTYPE array).
     Returns ISECT_DTYPE results (and per-ray stats: nodes visited, triangles
-    tested, deepest stack slot)."""
+    tested, deepest stack slot).  threads > 1 traces rays in parallel on host
+    threads (same outputs)."""
     L, R = _f32(L), _f32(R)
     meta = np.ascontiguousarray(meta, dtype=np.int32)
     V = _f32(vertices)
@@ -264,7 +267,8 @@ def obvh_intersect(L, R, meta, vertices, faces, primitives, rays, with_stats: bo
     n = rays.shape[0]
     out = np.zeros(n, dtype=ISECT_DTYPE)
     st = np.zeros((n, 3), dtype=np.uint32)
-    lib().oracle_obvh_intersect(_p(L), _p(R), _p(meta), _p(V), _p(F), _p(P), _p(rays), n, _p(out), _p(st))
+    lib().oracle_obvh_intersect_mt(_p(L), _p(R), _p(meta), _p(V), _p(F), _p(P), _p(rays), n, _p(out), _p(st),
+                                   int(threads))
     return (out, st) if with_stats else out
 
 

@@ -142,6 +142,10 @@ int oracle_obvh_pack(const onode* nodes, const omat4* mats, uint32_t node_count,
 int oracle_obvh_intersect(const float* left_T, const float* right_T, const int32_t* meta, const float* V,
                           const uint32_t* F, const uint32_t* prims, const oray* rays, uint32_t n, oisect* out,
                           uint32_t* stats);
+/* The same on `threads` host threads (rays in parallel, same results). */
+int oracle_obvh_intersect_mt(const float* left_T, const float* right_T, const int32_t* meta, const float* V,
+                             const uint32_t* F, const uint32_t* prims, const oray* rays, uint32_t n, oisect* out,
+                             uint32_t* stats, int threads);
 
 /* kr_ray_unit_aabb_intersect -> (t0, t1); kr_ray_triangle_intersect -> (b1, b2, t). */
 void oracle_ray_unit_box(const float* rd, const float* ro, float* out2);

@@ -164,11 +164,27 @@ static ov3 vert(const float* V, uint32_t i) {
     return p;
 }
 
-int oracle_obvh_intersect(const float* left_T, const float* right_T, const int32_t* meta, const float* V,
-                          const uint32_t* F, const uint32_t* prims, const oray* rays, uint32_t n, oisect* out,
-                          uint32_t* stats) {
-    int status = 0;
-    for (uint32_t r = 0; r < n; ++r) {
+typedef struct {
+    const float *left_T, *right_T;
+    const int32_t* meta;
+    const float* V;
+    const uint32_t *F, *prims;
+    const oray* rays;
+    oisect* out;
+    uint32_t* stats;
+} trace_ctx;
+
+/* One ray (the body of oracle_obvh_intersect). */
+static void trace_one(void* p, uint32_t r, int tid) {
+    const trace_ctx* c = (const trace_ctx*)p;
+    const float *left_T = c->left_T, *right_T = c->right_T, *V = c->V;
+    const int32_t* meta = c->meta;
+    const uint32_t *F = c->F, *prims = c->prims;
+    const oray* rays = c->rays;
+    oisect* out = c->out;
+    uint32_t* stats = c->stats;
+    (void)tid;
+    {
         const ov3 org = rays[r].origin, dir = rays[r].direction;
         float hitT = rays[r].tmax;
         uint32_t hit = 0xFFFFFFFFu;
@@ -244,12 +260,27 @@ int oracle_obvh_intersect(const float* left_T, const float* right_T, const int32
             o->bx = bx;
             o->by = by;
         }
-        if (overflow) status = -1;
         if (stats) {
             stats[3 * (size_t)r] = visits;
             stats[3 * (size_t)r + 1] = tests;
             stats[3 * (size_t)r + 2] = overflow ? 0xFFFFFFFFu : deepest;
         }
     }
+}
+
+int oracle_obvh_intersect(const float* left_T, const float* right_T, const int32_t* meta, const float* V,
+                          const uint32_t* F, const uint32_t* prims, const oray* rays, uint32_t n, oisect* out,
+                          uint32_t* stats) {
+    return oracle_obvh_intersect_mt(left_T, right_T, meta, V, F, prims, rays, n, out, stats, 1);
+}
+
+int oracle_obvh_intersect_mt(const float* left_T, const float* right_T, const int32_t* meta, const float* V,
+                             const uint32_t* F, const uint32_t* prims, const oray* rays, uint32_t n, oisect* out,
+                             uint32_t* stats, int threads) {
+    trace_ctx c = {left_T, right_T, meta, V, F, prims, rays, out, stats};
+    oracle_parallel_for(n, 64, threads, trace_one, &c);
+    int status = 0;
+    for (uint32_t r = 0; r < n; ++r)
+        if (out[r].instance == 0xFFFFFFFFu) status = -1;
     return status;
 }

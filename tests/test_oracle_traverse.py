@@ -155,6 +155,25 @@ def test_closest_hit_matches_brute_force(oracle_lib, kind, rays_kind):
     assert np.allclose(np.linalg.norm(out["normal"][h], axis=1), 1.0, atol=1e-6)
 
 
+@pytest.mark.parametrize("threads", [2, 7])
+def test_threaded_traversal_equals_sequential(oracle_lib, threads):
+    """The all-cores timing run traces the same hits and statistics, an
+    overflowing ray included."""
+    V, F, nodes, parents, counts, mats = lbvh_tree(oracle_lib, "hair")
+    L, R, meta = oracle_lib.obvh_pack(nodes, mats)
+    rays = synth.random_rays(V, 3000, seed=11)
+    seq, st1 = oracle_lib.obvh_intersect(L, R, meta, V, F, None, rays, with_stats=True)
+    par, stn = oracle_lib.obvh_intersect(L, R, meta, V, F, None, rays, with_stats=True, threads=threads)
+    assert seq.tobytes() == par.tobytes()
+    assert np.array_equal(st1, stn)
+    cL, cR, cmeta, cV, cF = chain_tree(64)
+    cr = chain_rays()
+    a = oracle_lib.obvh_intersect(cL, cR, cmeta, cV, cF, None, cr)
+    b = oracle_lib.obvh_intersect(cL, cR, cmeta, cV, cF, None, cr, threads=threads)
+    assert a.tobytes() == b.tobytes()
+    assert (b["instance"] == STACK_OVERFLOW).all()
+
+
 # ------------------------------------------------- the reference's loops --
 
 def reference_loops(oracle_lib, L, R, meta, V, F, prims, ray):
