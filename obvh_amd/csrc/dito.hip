@@ -251,19 +251,19 @@ __global__ __launch_bounds__(kBlock) void dito_pass1(P pts, const uint64_t* offs
     }
 }
 
-// ------------------------------------------------ candidate search (1 lane) --
+// ----------------------------------------- candidate search (one wave) --
 //
 // dito.cpp:347-418 over a 14-entry point list `sel`.  Sets of n <= 14 points
 // use their n points followed by copies of sel[0]: a copy never wins a strict
 // comparison against the first occurrence, so every result equals the
 // n-point search.  Candidate ids: 0..2 base triangle, 3..11 upper tetra,
-// 12..20 lower tetra.
-
-struct Search {
-    float best;
-    f3 b0, b1, b2;
-    int best_id;
-};
+// 12..20 lower tetra.  Every lane computes the common prefix (extreme pair,
+// base triangle, normal, apexes); lane c < 21 evaluates candidate c with the
+// same operations as the sequential scan.  The scan keeps the first candidate
+// whose quality is strictly below the best so far, starting from the AABB's:
+// that is the lexicographic minimum of (quality, id) over the candidates,
+// kept when it is strictly below the AABB's quality; a NaN quality or an
+// absent candidate (apex not valid) never wins.
 
 __device__ __forceinline__ void ext_projs(const f3* sel, f3 d, float& mn, float& mx) {
     float p = dot3(sel[0], d);
@@ -277,28 +277,41 @@ __device__ __forceinline__ void ext_projs(const f3* sel, f3 d, float& mn, float&
     }
 }
 
-__device__ __forceinline__ void eval_triangle(const f3* sel, Search& s, int id0, f3 n, f3 e0, f3 e1, f3 e2) {
-    float nmin, nmax;
-    ext_projs(sel, n, nmin, nmax);
-    const float leny = nmax - nmin;
-    const f3 e[3] = {e0, e1, e2};
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const f3 m = dm::cross(e[k], n);
-        float emin, emax, mmin, mmax;
-        ext_projs(sel, e[k], emin, emax);
-        ext_projs(sel, m, mmin, mmax);
-        const float q = quality(f3{emax - emin, leny, mmax - mmin});
-        if (q < s.best) {
-            s.best = q;
-            s.b0 = e[k];
-            s.b1 = n;
-            s.b2 = m;
-            s.best_id = id0 + k;
+// Quality of candidate c (dito.cpp:309-345 for one edge of one triangle), its
+// axes in b0..b2; +inf when the candidate does not exist.
+__device__ __forceinline__ float eval_candidate(const f3* sel, int c, f3 n, f3 e0, f3 e1, f3 e2, f3 p0, f3 p1,
+                                                f3 p2, f3 q0, f3 q1, bool q0v, bool q1v, f3& b0, f3& b1, f3& b2) {
+    const int face = c / 3, k = c - 3 * face;
+    f3 nf = n, ea = e0, eb = e1, ec = e2;
+    if (face > 0) {
+        const bool up = face <= 3;
+        if (up ? !q0v : !q1v) return FLT_MAX * 2.0f;
+        const f3 q = up ? q0 : q1;
+        const f3 f0 = dm::normalize(sub3(q, p0));
+        const f3 f1 = dm::normalize(sub3(q, p1));
+        const f3 f2 = dm::normalize(sub3(q, p2));
+        const int fi = (face - 1) % 3;  // dito.cpp:394-417
+        if (fi == 0) {
+            nf = dm::normalize(dm::cross(f1, e0)); ea = e0; eb = f1; ec = f0;
+        } else if (fi == 1) {
+            nf = dm::normalize(dm::cross(f2, e1)); ea = e1; eb = f2; ec = f1;
+        } else {
+            nf = dm::normalize(dm::cross(f0, e2)); ea = e2; eb = f0; ec = f2;
         }
     }
+    const f3 e = k == 0 ? ea : (k == 1 ? eb : ec);
+    float nmin, nmax, emin, emax, mmin, mmax;
+    ext_projs(sel, nf, nmin, nmax);
+    const f3 m = dm::cross(e, nf);
+    ext_projs(sel, e, emin, emax);
+    ext_projs(sel, m, mmin, mmax);
+    b0 = e;
+    b1 = nf;
+    b2 = m;
+    return quality(f3{emax - emin, nmax - nmin, mmax - mmin});
 }
 
+// Run by every lane of a wave; the results are uniform.
 __device__ void dito_candidate_search(const float mn[7], const float mx[7], const f3 minv[7], const f3 maxv[7],
                                       const f3 sel[14], SetState& st, obvh_dito_trace& tr) {
     const float eps = 0.000001f;
@@ -307,13 +320,6 @@ __device__ void dito_candidate_search(const float mn[7], const float mx[7], cons
     st.almid[0] = almid.x; st.almid[1] = almid.y; st.almid[2] = almid.z;
     st.allen[0] = allen.x; st.allen[1] = allen.y; st.allen[2] = allen.z;
     st.alval = quality(allen);
-
-    Search s;
-    s.best = st.alval;
-    s.b0 = f3{1, 0, 0};
-    s.b1 = f3{0, 1, 0};
-    s.b2 = f3{0, 0, 1};
-    s.best_id = -1;
 
     int kstar = 0;
     float dbest = dm::sqdist(maxv[0], minv[0]);
@@ -368,7 +374,6 @@ __device__ void dito_candidate_search(const float mn[7], const float mx[7], cons
     const f3 e1 = dm::normalize(sub3(p1, p2));
     const f3 e2 = dm::normalize(sub3(p2, p0));
     const f3 n = dm::normalize(dm::cross(e1, e0));
-    eval_triangle(sel, s, 0, n, e0, e1, e2);
 
     float pr = dot3(sel[0], n);
     float lo = pr, hi = pr;
@@ -387,31 +392,40 @@ __device__ void dito_candidate_search(const float mn[7], const float mx[7], cons
     tr.q1_sel = ilo;
     tr.q0_valid = q0v;
     tr.q1_valid = q1v;
-    if (q0v) {
-        const f3 f0 = dm::normalize(sub3(q0, p0));
-        const f3 f1 = dm::normalize(sub3(q0, p1));
-        const f3 f2 = dm::normalize(sub3(q0, p2));
-        eval_triangle(sel, s, 3, dm::normalize(dm::cross(f1, e0)), e0, f1, f0);
-        eval_triangle(sel, s, 6, dm::normalize(dm::cross(f2, e1)), e1, f2, f1);
-        eval_triangle(sel, s, 9, dm::normalize(dm::cross(f0, e2)), e2, f0, f2);
+
+    const int lane = lane_id();
+    const float INF = FLT_MAX * 2.0f;
+    f3 b0 = f3{1, 0, 0}, b1 = f3{0, 1, 0}, b2 = f3{0, 0, 1};
+    float q = INF;
+    if (lane < 21) q = eval_candidate(sel, lane, n, e0, e1, e2, p0, p1, p2, q0, q1, q0v != 0, q1v != 0, b0, b1, b2);
+    if (!(q == q)) q = INF;  // NaN never compares below anything
+    int id = lane;
+#pragma unroll
+    for (int m = 1; m < kWave; m <<= 1) {
+        const float oq = __shfl_xor(q, m, kWave);
+        const int oid = __shfl_xor(id, m, kWave);
+        if (oq < q || (oq == q && oid < id)) { q = oq; id = oid; }
     }
-    if (q1v) {
-        const f3 g0 = dm::normalize(sub3(q1, p0));
-        const f3 g1 = dm::normalize(sub3(q1, p1));
-        const f3 g2 = dm::normalize(sub3(q1, p2));
-        eval_triangle(sel, s, 12, dm::normalize(dm::cross(g1, e0)), e0, g1, g0);
-        eval_triangle(sel, s, 15, dm::normalize(dm::cross(g2, e1)), e1, g2, g1);
-        eval_triangle(sel, s, 18, dm::normalize(dm::cross(g0, e2)), e2, g0, g2);
+    int best_id = -1;
+    if (q < st.alval) {  // strictly better than the AABB (and than every earlier candidate)
+        best_id = id;
+        b0 = f3{__shfl(b0.x, id), __shfl(b0.y, id), __shfl(b0.z, id)};
+        b1 = f3{__shfl(b1.x, id), __shfl(b1.y, id), __shfl(b1.z, id)};
+        b2 = f3{__shfl(b2.x, id), __shfl(b2.y, id), __shfl(b2.z, id)};
+    } else {
+        b0 = f3{1, 0, 0};
+        b1 = f3{0, 1, 0};
+        b2 = f3{0, 0, 1};
     }
-    st.ax[0] = s.b0.x; st.ax[1] = s.b0.y; st.ax[2] = s.b0.z;
-    st.ax[3] = s.b1.x; st.ax[4] = s.b1.y; st.ax[5] = s.b1.z;
-    st.ax[6] = s.b2.x; st.ax[7] = s.b2.y; st.ax[8] = s.b2.z;
+    st.ax[0] = b0.x; st.ax[1] = b0.y; st.ax[2] = b0.z;
+    st.ax[3] = b1.x; st.ax[4] = b1.y; st.ax[5] = b1.z;
+    st.ax[6] = b2.x; st.ax[7] = b2.y; st.ax[8] = b2.z;
     st.mode = 0;
     tr.degenerate = 0;
-    tr.best = s.best_id;
+    tr.best = best_id;
 }
 
-// One wave per set: reduce the tile partials, then lane 0 searches.
+// One wave per set: reduce the tile partials, then the wave searches.
 template <class P>
 __global__ __launch_bounds__(kBlock) void dito_cand(P pts, const uint64_t* offs, uint32_t nsets, DitoWs ws,
                                                     obvh_dito_trace* trace) {
@@ -443,7 +457,6 @@ __global__ __launch_bounds__(kBlock) void dito_cand(P pts, const uint64_t* offs,
     for (int side = 0; side < 14; ++side) {
         if (side < 7) wave_pair<true>(v[side], ix[side]); else wave_pair<false>(v[side], ix[side]);
     }
-    if (lane != 0) return;
     const uint64_t n = send - sbeg;
 #pragma unroll
     for (int side = 0; side < 14; ++side)
@@ -473,8 +486,10 @@ __global__ __launch_bounds__(kBlock) void dito_cand(P pts, const uint64_t* offs,
     }
     SetState st = {};
     dito_candidate_search(mn, mx, minv, maxv, sel, st, tr);
-    ws.state[s] = st;
-    trace[s] = tr;
+    if (lane == 0) {
+        ws.state[s] = st;
+        trace[s] = tr;
+    }
 }
 
 // --------------------------------------------------------------- pass 2 --
