@@ -21,9 +21,11 @@
 //            lexicographically (ties keep the lower index): order free, and
 //            equal to the reference's sequential strict-compare scan.
 //            Output: 14 pairs per tile.
-//  cand      one wave per set reduces its tile partials, then one lane runs
-//            the candidate search (base triangle + ditetrahedron, up to 21
-//            axis triples) over a 14-entry point array held in registers.
+//  cand      one wave per set reduces its tile partials, then runs the
+//            candidate search (base triangle + ditetrahedron, up to 21 axis
+//            triples) over a 14-entry point array held in registers: every
+//            lane computes the common prefix, 21 lanes evaluate one axis
+//            triple each, and a (quality, id) minimum picks the winner.
 //  pass 2    the same tiles project every point on the 3 chosen axes (fused
 //            min/max of 3 dot products), 6 floats per tile.
 //  final     one wave per set reduces and finalises the box.

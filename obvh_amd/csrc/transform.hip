@@ -3,9 +3,10 @@
 // Method of the reference GPU transform (src/ads/common/cuda/transforms.cu
 // :213-964), restructured for CDNA4:
 //
-//  climb_leaves  one lane per leaf, 512-leaf workgroups: the leaf OBB and
-//                its matrix (transforms.cu:803-834), the leaf's 7-slab k-DOP
-//                (3 vertices, first vertex wins ties), then a bottom-up climb
+//  leaf_boxes    the leaf OBB and its matrix (transforms.cu:803-834), on a
+//                second stream beside the top-of-tree climb.
+//  climb_leaves  one lane per leaf, 512-leaf workgroups: the leaf's 7-slab
+//                k-DOP (3 vertices, first vertex wins ties), then a bottom-up climb
 //                in which the second child to arrive at a parent merges the
 //                two k-DOPs.  Arrivals at parents inside the workgroup's range
 //                are counted in LDS; the rest use a 64-bit global arrival
@@ -18,13 +19,18 @@
 //                to the left child and the selected vertex ids do not depend
 //                on scheduling.  k-DOPs carry vertex ids, not positions
 //                (7 x {min, id, max, id} = 112 B per node).
+//  list_wide     the internal nodes whose span holds a whole aligned refit
+//                tile, for the refit's ancestor lookup.
 //  candidates    one lane per internal node: gather the 14 extremal vertices
 //                and run the base-triangle + ditetrahedron search in
 //                registers (transforms.cu:413-756).
 //  refit         exact projection range of every subtree vertex on each
 //                node's candidate axes (transforms.cu:836-900): per-lane walks
 //                below each wave's common ancestor, DPP wave reductions above
-//                it, LDS hash per workgroup, filtered global atomics.
+//                it (both reading the tile's own nodes from an LDS copy), a
+//                tile-wide pass over the tile's distinct vertices for the
+//                ancestors of the tile (found in the wide-node list), LDS
+//                keys per workgroup, filtered global atomics.
 //  finalize      one lane per internal node: OBB vs AABB by half area, extent
 //                floor, inverse(T * R * S) (transforms.cu:905-964).
 //
