@@ -121,8 +121,9 @@ class Server:
         self._fs ^= 1
         for r in fits:
             p = r.points
-            if (p.dim() != 2 or p.shape[0] != 3 or p.dtype != torch.float32 or not p.is_contiguous()
-                    or not (p.is_pinned() or p.is_cuda)):
+            # pinned host or device memory is checked per set by obvh_gather_sets
+            # (one pointer query each; a second one here cost as much again)
+            if p.dim() != 2 or p.shape[0] != 3 or p.dtype != torch.float32 or not p.is_contiguous():
                 raise ValueError(f"request {r.rid}: points must be a contiguous float32 (3, n) tensor "
                                  "in pinned host or device memory")
         sizes = np.array([int(r.points.shape[1]) for r in fits], dtype=np.int64)
