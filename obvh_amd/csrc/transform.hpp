@@ -26,7 +26,7 @@ struct TreeWs {
     Cand* cand;        // per internal node
     int32_t* range;    // 6 per internal node: order-preserving keys, min b0..b2, max b0..b2
     // Wide nodes: internal nodes whose contiguous leaf span holds at least
-    // one whole aligned refit tile, listed by the candidates kernel (span and
+    // one whole aligned refit tile, listed by the list_wide kernel (span and
     // node id; at most kWideMax).  wide_count counts every such node (it may
     // exceed the list), wide_gap is nonzero when a node spanning a tile or
     // more has non-contiguous leaves (the list then does not describe the
