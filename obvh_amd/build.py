@@ -1,7 +1,10 @@
 """Build the HIP shared library obvh_amd/libobvh.so for gfx950.
 
 Every HIP source is compiled with hipcc in parallel and linked into one
-shared library that exports the C ABI of include/obvh/obvh.h.
+shared library that exports the C ABI of include/obvh/obvh.h.  The two
+intersector plugins of include/obvh/korangar_ads.h (liblbvh_gpu.so,
+libatrbvh_gpu.so: one source, two builder settings) link against it and find
+it next to themselves (RPATH $ORIGIN).
 
 Floating point flags are part of the parity contract with the CPU oracle:
   -ffp-contract=off                          no FMA contraction
@@ -23,6 +26,9 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libobvh.so")
 ARCH = os.environ.get("OBVH_ARCH", "gfx950")
+
+# plugin library name -> value of OBVH_ADS_ATRBVH for ads_plugin.hip
+PLUGINS = {"lbvh_gpu": 0, "atrbvh_gpu": 1}
 
 SOURCES = ["abi.hip", "dito.hip", "transform.hip", "lbvh.hip", "collapse.hip", "atrbvh.hip", "traverse.hip", "upload.hip"]
 
@@ -57,19 +63,43 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(verbose: bool = False) -> str:
-    """Compile every HIP source and link libobvh.so (always rebuilds)."""
-    os.makedirs(BUILD, exist_ok=True)
-    with cf.ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 4)) as ex:
-        objs = list(ex.map(_compile, SOURCES))
-    tmp = LIB + ".tmp"
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp]
+def plugin_path(name: str) -> str:
+    return os.path.join(HERE, f"lib{name}.so")
+
+
+def _compile_plugin(name: str) -> str:
+    obj = os.path.join(BUILD, f"ads_plugin_{name}.o")
+    cmd = [hipcc(), *FLAGS, f"-DOBVH_ADS_ATRBVH={PLUGINS[name]}", "-c", os.path.join(CSRC, "ads_plugin.hip"),
+           "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError(f"link failed:\n{r.stderr}")
-    os.replace(tmp, LIB)
+        raise RuntimeError(f"hipcc failed on ads_plugin.hip ({name}):\n{r.stderr}")
+    return obj
+
+
+def _link(objs, out, extra=()) -> None:
+    tmp = out + ".tmp"
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, *extra, "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link of {os.path.basename(out)} failed:\n{r.stderr}")
+    os.replace(tmp, out)
+
+
+def build(verbose: bool = False) -> str:
+    """Compile every HIP source, link libobvh.so, then the two intersector
+    plugins against it (always rebuilds)."""
+    os.makedirs(BUILD, exist_ok=True)
+    with cf.ThreadPoolExecutor(max_workers=min(len(SOURCES) + len(PLUGINS), os.cpu_count() or 4)) as ex:
+        lib_objs = ex.map(_compile, SOURCES)
+        plugin_objs = {name: ex.submit(_compile_plugin, name) for name in PLUGINS}
+        objs = list(lib_objs)
+        plugin_objs = {name: f.result() for name, f in plugin_objs.items()}
+    _link(objs, LIB)
+    for name, obj in plugin_objs.items():
+        _link([obj], plugin_path(name), ("-L" + HERE, "-lobvh", "-Wl,-rpath,$ORIGIN"))
     if verbose:
-        print(f"built {LIB}", file=sys.stderr)
+        print(f"built {LIB} and {', '.join(os.path.basename(plugin_path(n)) for n in PLUGINS)}", file=sys.stderr)
     return LIB
 
 
