@@ -4,6 +4,7 @@ PREFERRED_QUERY_TYPE, CREATE, INIT, DESTROY, COMMIT's scene checks).  The
 reference's behaviour is src/ads/lbvh_gpu/module.c:6-41 and lbvh.cu:1764-1800
 (empty scene message)."""
 import ctypes as C
+import os
 import subprocess
 
 import numpy as np
@@ -99,3 +100,48 @@ def test_actions_without_a_context_fail():
     ads = K.Ads()
     for a in ("INIT", "COMMIT", "INTERSECTION_QUERY"):
         assert "no context" in K.message(fn(C.byref(ads), None, K.ACTION[a]))
+
+
+HOST_C = r"""
+#include <dlfcn.h>
+#include <stdio.h>
+#include <string.h>
+#include "obvh/korangar_ads.h"
+
+/* the reference's loader, src/common/ads.c:6-37 */
+int main(int argc, char** argv) {
+    void* lib = dlopen(argv[1], RTLD_NOW);
+    if (!lib) { printf("dlopen: %s\n", dlerror()); return 1; }
+    kr_ads ads;
+    memset(&ads, 0, sizeof(ads));
+    ads.callback = (kr_ads_callback)dlsym(lib, "korangar_ads_action");
+    if (!ads.callback) return 2;
+    ads.context = ads.callback(NULL, NULL, KR_ADS_ACTION_CREATE);
+    ads.library = lib;
+    kr_descriptor d[2] = {{"use_obbs", "y"}, {"collapse", "y"}};
+    kr_descriptor_container c = {d, 2};
+    if (ads.callback(&ads, &c, KR_ADS_ACTION_INIT) != NULL) return 3;
+    kr_scene scene;
+    memset(&scene, 0, sizeof(scene));
+    const char* e = (const char*)ads.callback(&ads, &scene, KR_ADS_ACTION_COMMIT);
+    printf("%s|%s|%d\n", (const char*)ads.callback(&ads, NULL, KR_ADS_ACTION_NAME), e ? e : "(null)",
+           (int)(size_t)ads.callback(&ads, NULL, KR_ADS_ACTION_PREFERRED_QUERY_TYPE));
+    ads.callback(&ads, NULL, KR_ADS_ACTION_DESTROY);
+    return ads.context == NULL ? 0 : 4;
+}
+"""
+
+
+@pytest.mark.parametrize("name", sorted(PLUGINS))
+def test_c_host_loads_the_plugin_like_the_reference(tmp_path, name):
+    """A C program using only include/obvh/korangar_ads.h and dlopen drives
+    the plugin the way the reference's application does."""
+    src = tmp_path / "host.c"
+    src.write_text(HOST_C)
+    exe = tmp_path / "host"
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    subprocess.run(["gcc", "-std=c11", "-Wall", "-Werror", "-I", os.path.join(root, "include"), str(src), "-o",
+                    str(exe), "-ldl"], check=True)
+    r = subprocess.run([str(exe), K.build.plugin_path(name)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.strip() == f"{PLUGINS[name]}|The scene has no intersectable primitives|2"
