@@ -295,3 +295,59 @@ def test_reference_cpu_transform_matches_dito_of_subtree(oracle_lib, tree):
                 queue += [int(nodes[u]["a"]), int(nodes[u]["b"])]
         P = V[F[nodes["a"][order]][:, :3].ravel()]
         assert np.array_equal(obbs[v], oracle_lib.points_obb(P))
+
+
+def _obb_area(o):
+    e = np.asarray(o, np.float64)[..., 12:15]
+    return 2 * (e[..., 0] * e[..., 1] + e[..., 0] * e[..., 2] + e[..., 1] * e[..., 2])
+
+
+@pytest.mark.parametrize("kind", ["ico", "hair"])
+def test_transform_against_dito_of_each_subtree(oracle_lib, kind):
+    """Cross-check of the transform oracle against an independent
+    computation: kr_points_obb (DiTO_14, pinned in test_oracle_dito.py) of
+    every subtree's vertices.  The two reference methods differ in three
+    places, so the check is statistical where they may differ and exact
+    where they must agree:
+      * a base triangle DiTO finds degenerate (points on a line): DiTO builds
+        a line-aligned box (dito.cpp:524), the GPU transform keeps the
+        node's AABB (transforms.cu:717-723) -- here the AABB exactly;
+      * ties between distinct points are broken in DFS order here, in DiTO's
+        point order there, and DiTO normalises by reciprocal-multiply
+        (dito.cpp:99-103) where the transform divides: a near-tied candidate
+        choice can flip, so only most nodes agree to 1e-4;
+      * a subtree of few points runs DiTO's search over all of them
+        (dito.cpp:519-520), the transform over its 14 extremal points:
+        only nodes with more than 14 distinct vertices are compared.
+    A wrong sign, index or dropped candidate in the transform shifts the
+    median and the mean area ratio far beyond the bounds below."""
+    V, F = (synth.icosphere(14, seed=5) if kind == "ico" else synth.hairball(tubes=120, seed=6))
+    nodes, parents, *_ = oracle_lib.lbvh_build(V, F)
+    obbs, _ = oracle_lib.obb_tree(nodes, parents, V, F)
+    I = len(F) - 1
+    rel, n_line = [], 0
+    for v in range(I):
+        queue, order = [v], []
+        while queue:
+            u = queue.pop(0)
+            if u >= I:
+                order.append(u)
+            else:
+                queue += [int(nodes[u]["a"]), int(nodes[u]["b"])]
+        P = V[F[nodes["a"][order]][:, :3].ravel()]
+        ref, tr = oracle_lib.points_obb(P, trace=True)
+        if int(tr["degenerate"]) == 2:
+            n_line += 1
+            ext = np.maximum((P.max(0) - P.min(0)) * np.float32(0.5), np.float32(0.001))
+            assert np.array_equal(obbs[v][3:12], np.eye(3, dtype=np.float32).ravel()), v
+            assert np.allclose(obbs[v][12:15], ext, rtol=1e-6, atol=0), v
+            continue
+        if int(tr["degenerate"]) == 0 and len(np.unique(P, axis=0)) > 14:
+            rel.append(_obb_area(obbs[v]) / _obb_area(ref) - 1.0)
+    rel = np.array(rel)
+    assert rel.size > I // 20  # the upper levels of the tree: about an eighth of the nodes
+    assert np.median(np.abs(rel)) < 1e-5
+    assert np.mean(np.abs(rel) <= 1e-4) >= 0.9
+    assert abs(np.mean(rel)) < 0.01
+    if kind == "hair":
+        assert n_line > 0  # thin tubes do reach the line case
