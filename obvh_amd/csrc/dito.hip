@@ -13,11 +13,11 @@
 //            tile per workgroup by default).  Lane l of a
 //            wave reads points chunk + j*64 + l, j < 8 (coalesced SoA or AoS),
 //            forms the 7 slab projections, and keeps per slab side the running
-//            extreme with the index where it was first reached.  The 8-point
-//            group extreme costs 7 min/max ops per side; only a strict
-//            improvement (rare once a few chunks are seen) searches the group
-//            for its first attaining point, so index bookkeeping costs one
-//            compare per 8 points.  (value, index) pairs reduce
+//            extreme with the index where it was first reached: one strict
+//            compare and two selects per point and side (a group-extreme
+//            scheme that searched the group on improvement ran nearly every
+//            search, since some lane of a wave improves on most groups, and
+//            issued 3.5x the VALU work of pass 2).  (value, index) pairs reduce
 //            lexicographically (ties keep the lower index): order free, and
 //            equal to the reference's sequential strict-compare scan.
 //            Output: 14 pairs per tile.
@@ -60,16 +60,6 @@ struct PtsAoS {
 
 __device__ __forceinline__ uint64_t set_off(const uint64_t* offs, uint64_t single, uint32_t s) {
     return offs ? offs[s] : (s ? single : 0);
-}
-
-__device__ __forceinline__ float slab_k(f3 p, int k) {
-    float pr[7];
-    slab7(p, pr);
-    float v = pr[0];
-#pragma unroll
-    for (int i = 1; i < 7; ++i)
-        if (k == i) v = pr[i];
-    return v;
 }
 
 // ---------------------------------------------------------------- setup --
@@ -178,45 +168,21 @@ __global__ __launch_bounds__(kBlock) void dito_pass1(P pts, const uint64_t* offs
                 const uint64_t idx = cb + (uint64_t)j * kWave;
                 p[j] = (full || idx < end) ? pts.load(idx) : f3{NaN, NaN, NaN};
             }
-            float gmn[7], gmx[7];
-            slab7(p[0], gmn);
+            // Per point and slab side a strict compare against the running
+            // extreme keeps the first point that reaches it, as the sequential
+            // scan does; !(v >= cur) also takes the first value, while cur is
+            // NaN.  NaN values (the short last chunk's missing points, or
+            // non-finite input) are skipped: a NaN is never selected.
+            const uint32_t i0 = (uint32_t)(cb - sbeg);
 #pragma unroll
-            for (int k = 0; k < 7; ++k) gmx[k] = gmn[k];
-#pragma unroll
-            for (int j = 1; j < kG; ++j) {
+            for (int j = 0; j < kG; ++j) {
                 float pr[7];
                 slab7(p[j], pr);
+                const uint32_t idx = i0 + (uint32_t)j * kWave;
 #pragma unroll
                 for (int k = 0; k < 7; ++k) {
-                    gmn[k] = fminf(gmn[k], pr[k]);  // NaN placeholders are ignored
-                    gmx[k] = fmaxf(gmx[k], pr[k]);
-                }
-            }
-            // Strict improvement (or first value): locate the first point of
-            // the group that attains it.  Rare after the first chunks.
-#pragma unroll
-            for (int k = 0; k < 7; ++k) {
-                if (gmn[k] < rmn[k] || (imn[k] == kNoIdx && gmn[k] == gmn[k])) {
-                    uint32_t jj = kG;
-#pragma unroll
-                    for (int j = kG - 1; j >= 0; --j)
-                        if (slab_k(p[j], k) == gmn[k]) jj = (uint32_t)j;
-                    rmn[k] = slab_k(p[0], k);
-#pragma unroll
-                    for (int j = 1; j < kG; ++j)
-                        if ((uint32_t)j == jj) rmn[k] = slab_k(p[j], k);
-                    imn[k] = (uint32_t)(cb + (uint64_t)jj * kWave - sbeg);
-                }
-                if (gmx[k] > rmx[k] || (imx[k] == kNoIdx && gmx[k] == gmx[k])) {
-                    uint32_t jj = kG;
-#pragma unroll
-                    for (int j = kG - 1; j >= 0; --j)
-                        if (slab_k(p[j], k) == gmx[k]) jj = (uint32_t)j;
-                    rmx[k] = slab_k(p[0], k);
-#pragma unroll
-                    for (int j = 1; j < kG; ++j)
-                        if ((uint32_t)j == jj) rmx[k] = slab_k(p[j], k);
-                    imx[k] = (uint32_t)(cb + (uint64_t)jj * kWave - sbeg);
+                    if (!(pr[k] >= rmn[k]) && pr[k] == pr[k]) { rmn[k] = pr[k]; imn[k] = idx; }
+                    if (!(pr[k] <= rmx[k]) && pr[k] == pr[k]) { rmx[k] = pr[k]; imx[k] = idx; }
                 }
             }
         }
