@@ -65,7 +65,7 @@ class Server:
     # PCIe gather overlap the previous batch on the other fit stream: a burst
     # of 4096 fits (2.8 GB) measured 72 K sets/s at 2^23..2^25 against
     # 60-69 K with 2^27 (two batches, the first gather waiting for the host
-    # to pack half the burst).
+    # to pack half the burst), before the gather's 16-byte reads (77 K).
     def __init__(self, hbm_budget: float = 200e9, tree_streams: int = 4, max_batch_points: int = 1 << 24,
                  keep_trees: bool = False):
         import torch
