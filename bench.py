@@ -344,8 +344,9 @@ def traversal_leg(V, F, Vd, Fd, nodes, mats, prims, label: str, steps: int, warm
 
 def transform_leg(V, F, label: str, steps: int, warmup: int, with_cpu: bool, builder: str = "lbvh",
                   trace: bool = False) -> dict:
-    """LBVH (or ATRBVH: treelet 7, 2 passes) of the mesh built on the GPU, then
-    `steps` timed transforms of the resident tree, then the collapse; optionally
+    """LBVH (or ATRBVH: treelet 7, 2 passes) of the mesh built on the GPU and
+    its build timed (mean of 5 LBVH builds, one ATRBVH build), then `steps`
+    timed transforms of the resident tree, then the collapse; optionally
     the CPU oracle on the same tree (bit-exact comparison and single-thread
     CPU throughput)."""
     import numpy as np
@@ -368,7 +369,16 @@ def transform_leg(V, F, label: str, steps: int, warmup: int, with_cpu: bool, bui
         build_ms = b0.elapsed_time(b1)
         print(f"[bench] atrbvh build {label}: {build_ms:.3f} ms device", file=sys.stderr, flush=True)
     else:
-        tree = ops.lbvh_build(Vd, Fd)
+        ops.lbvh_build(Vd, Fd)  # warm
+        torch.cuda.synchronize()
+        b0, b1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        b0.record()
+        for _ in range(5):
+            tree = ops.lbvh_build(Vd, Fd)
+        b1.record()
+        torch.cuda.synchronize()
+        build_ms = b0.elapsed_time(b1) / 5
+        print(f"[bench] lbvh build {label}: {build_ms:.3f} ms device", file=sys.stderr, flush=True)
     total = 2 * N - 1
     obbs = torch.empty((total, 15), dtype=torch.float32, device="cuda")
     mats = torch.empty((total, 16), dtype=torch.float32, device="cuda")
@@ -574,6 +584,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "cpu_reference_transform": main_leg.get("cpu_reference_transform"),
         "parity": parity,
+        "lbvh_build_ms": main_leg["build_ms"],
         "collapse_ms": main_leg["collapse_ms"],
         "sah": main_leg["sah"],
         "traversal": main_leg.get("traversal"),
