@@ -162,7 +162,8 @@ typedef struct {
     int32_t q0_sel;        /* max point along the base normal, or -1       */
     int32_t q1_sel;        /* min point along the base normal, or -1       */
     int32_t q0_valid, q1_valid;
-    int32_t degenerate;    /* 0 ok, 1 AABB (pair too close), 2 line-aligned, 3 empty */
+    int32_t degenerate;    /* 0 ok, 1 AABB (pair too close), 2 line-aligned, 3 empty,
+                              4 invalid offsets (decreasing or past total_points: NaN box) */
     int32_t best;          /* winning candidate 0..20, -1 = initial AABB axes */
     int32_t result_aabb;   /* 1 if the returned box is the axis-aligned one */
     int32_t pad_;
@@ -207,9 +208,14 @@ kr_error obvh_gather_sets(const float* const* host_sets, const float* const* set
                           const kr_size* offsets, const kr_size* offsets_table, u32 nsets,
                           float* soa, kr_size row_stride, void* stream);
 
-/* Transform with caller workspace, stream ordered.  Same semantics as
- * kr_cuda_bvh_obb_tree; `kdop_ids` (device, nullable) receives for every
- * node the 14 selected extremal vertex ids [min0..min6, max0..max6].      */
+/* Transform with caller workspace, stream ordered.  The kernels compute
+ * what kr_cuda_bvh_obb_tree computes; the returned value reports argument
+ * and launch errors only.  A malformed tree (the errors
+ * kr_cuda_bvh_obb_tree returns) is found on the device: it leaves NaN
+ * boxes and a nonzero status word in the workspace, which
+ * obvh_obb_tree_status_async copies out.  `kdop_ids` (device, nullable)
+ * receives for every node the 14 selected extremal vertex ids
+ * [min0..min6, max0..max6].                                              */
 size_t obvh_obb_tree_workspace_size(u32 primitive_count, u32 internal_count);
 kr_error obvh_obb_tree_async(
     const kr_bvh_node* nodes, const u32* parents,
@@ -217,6 +223,15 @@ kr_error obvh_obb_tree_async(
     kr_obb3* obbs, kr_mat4* transforms, u32* kdop_ids,
     u32 primitive_count, u32 internal_count,
     void* workspace, size_t workspace_bytes, void* stream);
+
+/* Status of the last obvh_obb_tree_async call on `workspace` (same
+ * primitive_count): a kernel on `stream` copies its device error word into
+ * *status (device or pinned host memory; 0 = success).
+ * obvh_obb_tree_status_string maps a nonzero word to the error message
+ * kr_cuda_bvh_obb_tree would return (NULL for 0).                        */
+kr_error obvh_obb_tree_status_async(const void* workspace, size_t workspace_bytes,
+                                    u32 primitive_count, u32* status, void* stream);
+kr_error obvh_obb_tree_status_string(u32 status);
 
 /* Host-side tree validation (parents/children consistent, single root at
  * node 0, every node reachable once).  Inputs are HOST arrays.  Returns

@@ -23,6 +23,8 @@ EXPORTS = [
     "obvh_gather_sets",
     "obvh_obb_tree_workspace_size",
     "obvh_obb_tree_async",
+    "obvh_obb_tree_status_async",
+    "obvh_obb_tree_status_string",
     "obvh_validate_tree_host",
     "obvh_lbvh_workspace_size",
     "obvh_lbvh_build",
@@ -92,6 +94,10 @@ def lib():
     L.obvh_obb_tree_workspace_size.restype = sz
     L.obvh_obb_tree_async.argtypes = [vp, vp, vp, vp, vp, vp, vp, u32, u32, vp, sz, vp]
     L.obvh_obb_tree_async.restype = C.c_char_p
+    L.obvh_obb_tree_status_async.argtypes = [vp, sz, u32, vp, vp]
+    L.obvh_obb_tree_status_async.restype = C.c_char_p
+    L.obvh_obb_tree_status_string.argtypes = [u32]
+    L.obvh_obb_tree_status_string.restype = C.c_char_p
     L.obvh_validate_tree_host.argtypes = [vp, vp, u32, u32]
     L.obvh_validate_tree_host.restype = C.c_char_p
     L.obvh_lbvh_workspace_size.argtypes = [u32]

@@ -279,6 +279,16 @@ kr_error kr_cuda_bvh_obb_tree(kr_bvh_node* nodes, const u32* parents, const u32*
     return nullptr;
 }
 
+kr_error obvh_obb_tree_status_async(const void* workspace, size_t workspace_bytes, u32 primitive_count,
+                                    u32* status, void* stream) {
+    if (!status) return fail("obvh_obb_tree_status_async: status is NULL");
+    if (primitive_count == 0) return fail("obvh_obb_tree_status_async: no tree has zero primitives");
+    TRY(obvh::tree_status_launch(workspace, workspace_bytes, primitive_count, status, (hipStream_t)stream));
+    return nullptr;
+}
+
+kr_error obvh_obb_tree_status_string(u32 status) { return obvh::tree_status_string(status); }
+
 kr_error obvh_validate_tree_host(const kr_bvh_node* nodes, const u32* parents, u32 primitive_count,
                                  u32 internal_count) {
     if (primitive_count == 0) return nullptr;

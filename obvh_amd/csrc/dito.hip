@@ -64,8 +64,14 @@ __device__ __forceinline__ uint64_t set_off(const uint64_t* offs, uint64_t singl
 
 // ---------------------------------------------------------------- setup --
 
+// A set whose offsets decrease or run past `total` is invalid: it gets no
+// tiles, a NaN box and degenerate = 4, so no later pass reads or writes
+// outside the point arrays or the carved tile arrays (`cap_tiles` entries).
+__device__ __forceinline__ bool set_valid(uint64_t b, uint64_t e, uint64_t total) { return b <= e && e <= total; }
+
 __global__ __launch_bounds__(1024) void dito_setup(const uint64_t* offs, uint64_t single_count,
-                                                   uint32_t nsets, uint32_t tile, DitoWs ws,
+                                                   uint32_t nsets, uint32_t tile, uint64_t total,
+                                                   uint32_t cap_tiles, DitoWs ws,
                                                    kr_obb3* out, obvh_dito_trace* trace) {
     __shared__ uint32_t carry;
     __shared__ uint32_t wsum[1024 / kWave];
@@ -79,9 +85,21 @@ __global__ __launch_bounds__(1024) void dito_setup(const uint64_t* offs, uint64_
         const uint32_t s = base + threadIdx.x;
         uint32_t nt = 0;
         if (s < nsets) {
-            const uint64_t n = set_off(offs, single_count, s + 1) - set_off(offs, single_count, s);
-            nt = (uint32_t)((n + tile - 1) / tile);
-            if (n == 0) {
+            const uint64_t b = set_off(offs, single_count, s), e = set_off(offs, single_count, s + 1);
+            const uint64_t n = e - b;
+            if (!set_valid(b, e, total)) {
+                const float nan = __builtin_nanf("");
+                kr_obb3 o;
+                o.mid = o.v0 = o.v1 = o.v2 = o.ext = kr_vec3{nan, nan, nan};
+                out[s] = o;
+                obvh_dito_trace t = {};
+                t.pair_k = t.p2_sel = t.q0_sel = t.q1_sel = -1;
+                t.best = -1;
+                t.degenerate = 4;
+                trace[s] = t;
+            } else if (n != 0) {
+                nt = (uint32_t)((n + tile - 1) / tile);
+            } else {
                 // DiTO_14 of an empty set (dito.cpp:498-502) + extent floor.
                 kr_obb3 o;
                 o.mid = kr_vec3{0, 0, 0};
@@ -115,7 +133,9 @@ __global__ __launch_bounds__(1024) void dito_setup(const uint64_t* offs, uint64_
         if (threadIdx.x == blockDim.x - 1) carry = incl;
         __syncthreads();
     }
-    if (threadIdx.x == 0) ws.first_tile[nsets] = carry;
+    // valid offsets need at most cap_tiles - 1 tiles; the clamp only guards
+    // the carved arrays
+    if (threadIdx.x == 0) ws.first_tile[nsets] = carry < cap_tiles ? carry : cap_tiles;
 }
 
 __global__ __launch_bounds__(256) void dito_tile_table(uint32_t nsets, DitoWs ws) {
@@ -395,12 +415,12 @@ __device__ void dito_candidate_search(const float mn[7], const float mx[7], cons
 
 // One wave per set: reduce the tile partials, then the wave searches.
 template <class P>
-__global__ __launch_bounds__(kBlock) void dito_cand(P pts, const uint64_t* offs, uint32_t nsets, DitoWs ws,
-                                                    obvh_dito_trace* trace) {
+__global__ __launch_bounds__(kBlock) void dito_cand(P pts, const uint64_t* offs, uint32_t nsets, uint64_t total,
+                                                    DitoWs ws, obvh_dito_trace* trace) {
     const uint32_t s = blockIdx.x * kWaves + wave_id();
     if (s >= nsets) return;
     const uint64_t sbeg = offs[s], send = offs[s + 1];
-    if (send == sbeg) return;
+    if (send == sbeg || !set_valid(sbeg, send, total)) return;
     const uint32_t ft = ws.first_tile[s], nt = ws.first_tile[s + 1] - ft;
     const int lane = lane_id();
     float v[14];
@@ -426,9 +446,25 @@ __global__ __launch_bounds__(kBlock) void dito_cand(P pts, const uint64_t* offs,
         if (side < 7) wave_pair<true>(v[side], ix[side]); else wave_pair<false>(v[side], ix[side]);
     }
     const uint64_t n = send - sbeg;
+    // The reference scan starts from point 0 and compares strictly
+    // (dito.cpp:116-173): a slab whose value at point 0 is NaN stays (NaN, 0),
+    // since no later compare against NaN succeeds; otherwise NaN values are
+    // never taken and the result is the (value, first index) minimum over
+    // the other points, which is what the reductions above computed.
+    {
+        float pr0[7];
+        slab7(pts.load(sbeg), pr0);
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            if (pr0[k] != pr0[k]) {
+                v[k] = v[7 + k] = pr0[k];
+                ix[k] = ix[7 + k] = 0;
+            }
+        }
+    }
 #pragma unroll
     for (int side = 0; side < 14; ++side)
-        if ((uint64_t)ix[side] >= n) ix[side] = 0;  // only reachable with NaN-only input: never index past the set
+        if ((uint64_t)ix[side] >= n) ix[side] = 0;  // not reachable (a slab that is NaN everywhere is NaN at point 0)
     obvh_dito_trace tr = {};
     tr.pair_k = tr.p2_sel = tr.q0_sel = tr.q1_sel = -1;
     tr.best = -1;
@@ -520,11 +556,13 @@ __global__ __launch_bounds__(kBlock) void dito_pass2(P pts, const uint64_t* offs
 
 // ---------------------------------------------------------------- final --
 
-__global__ __launch_bounds__(kBlock) void dito_final(const uint64_t* offs, uint32_t nsets, DitoWs ws,
-                                                     kr_obb3* out, obvh_dito_trace* trace) {
+template <class P>
+__global__ __launch_bounds__(kBlock) void dito_final(P pts, const uint64_t* offs, uint32_t nsets, uint64_t total,
+                                                     DitoWs ws, kr_obb3* out, obvh_dito_trace* trace) {
     const uint32_t s = blockIdx.x * kWaves + wave_id();
     if (s >= nsets) return;
-    if (offs[s + 1] == offs[s]) return;
+    const uint64_t sbeg = offs[s], send = offs[s + 1];
+    if (send == sbeg || !set_valid(sbeg, send, total)) return;
     const uint32_t ft = ws.first_tile[s], nt = ws.first_tile[s + 1] - ft;
     const float INF = FLT_MAX * 2.0f;
     float a[6] = {INF, INF, INF, -INF, -INF, -INF};
@@ -542,6 +580,16 @@ __global__ __launch_bounds__(kBlock) void dito_final(const uint64_t* offs, uint3
     const f3 b0 = f3{st.ax[0], st.ax[1], st.ax[2]};
     const f3 b1 = f3{st.ax[3], st.ax[4], st.ax[5]};
     const f3 b2 = f3{st.ax[6], st.ax[7], st.ax[8]};
+    // computeObbDimensions (dito.cpp:421-426) starts from point 0 with strict
+    // compares: a NaN projection of point 0 stays, later NaNs are skipped
+    // (fminf / fmaxf above skip them too).
+    if (st.mode != 1) {
+        const f3 p0 = pts.load(sbeg);
+        const float d[3] = {dot3(p0, b0), dot3(p0, b1), dot3(p0, b2)};
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            if (d[k] != d[k]) a[k] = a[3 + k] = d[k];
+    }
     const f3 bmn = f3{a[0], a[1], a[2]}, bmx = f3{a[3], a[4], a[5]};
     const f3 blen = sub3(bmx, bmn);
     kr_obb3 o;
@@ -607,13 +655,13 @@ uint32_t dito_auto_tile(uint32_t nsets, uint64_t total) {
 }
 
 template <class P>
-static void launch_passes(P p, const uint64_t* offs, uint32_t nsets, uint32_t tile, uint32_t workers, DitoWs ws,
-                          kr_obb3* out, obvh_dito_trace* tr, hipStream_t stream) {
+static void launch_passes(P p, const uint64_t* offs, uint32_t nsets, uint64_t total, uint32_t tile, uint32_t workers,
+                          DitoWs ws, kr_obb3* out, obvh_dito_trace* tr, hipStream_t stream) {
     const dim3 set_grid((nsets + kWaves - 1) / kWaves);
     hipLaunchKernelGGL(dito_pass1<P>, dim3(workers), dim3(kBlock), 0, stream, p, offs, nsets, tile, ws);
-    hipLaunchKernelGGL(dito_cand<P>, set_grid, dim3(kBlock), 0, stream, p, offs, nsets, ws, tr);
+    hipLaunchKernelGGL(dito_cand<P>, set_grid, dim3(kBlock), 0, stream, p, offs, nsets, total, ws, tr);
     hipLaunchKernelGGL(dito_pass2<P>, dim3(workers), dim3(kBlock), 0, stream, p, offs, nsets, tile, ws);
-    hipLaunchKernelGGL(dito_final, set_grid, dim3(kBlock), 0, stream, offs, nsets, ws, out, tr);
+    hipLaunchKernelGGL(dito_final<P>, set_grid, dim3(kBlock), 0, stream, p, offs, nsets, total, ws, out, tr);
 }
 
 kr_error dito_launch(const float* x, const float* y, const float* z, const float* xyz,
@@ -633,12 +681,14 @@ kr_error dito_launch(const float* x, const float* y, const float* z, const float
     // workgroups looping over tiles)
     if (workers == 0) workers = (uint32_t)(T < 65536 ? T : 65536);
     if (workers > T) workers = (uint32_t)T;
-    hipLaunchKernelGGL(dito_setup, dim3(1), dim3(1024), 0, stream, offsets, single_count, nsets, tile, ws, out, tr);
+    if (T > 0xFFFFFFF0ull) return "obvh_dito: too many tiles";
+    hipLaunchKernelGGL(dito_setup, dim3(1), dim3(1024), 0, stream, offsets, single_count, nsets, tile, total,
+                       (uint32_t)T, ws, out, tr);
     uint32_t tb = (uint32_t)((T + 255) / 256);
     if (tb > 4096) tb = 4096;
     hipLaunchKernelGGL(dito_tile_table, dim3(tb), dim3(256), 0, stream, nsets, ws);
-    if (xyz) launch_passes(PtsAoS{xyz}, offs, nsets, tile, workers, ws, out, tr, stream);
-    else launch_passes(PtsSoA{x, y, z}, offs, nsets, tile, workers, ws, out, tr, stream);
+    if (xyz) launch_passes(PtsAoS{xyz}, offs, nsets, total, tile, workers, ws, out, tr, stream);
+    else launch_passes(PtsSoA{x, y, z}, offs, nsets, total, tile, workers, ws, out, tr, stream);
     OBVH_HIP_TRY(hipGetLastError());
     return nullptr;
 }

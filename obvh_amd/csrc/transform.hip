@@ -960,14 +960,34 @@ __device__ __forceinline__ void fold_tile(RefitTile& R, int32_t* range, uint32_t
     fold_global(&range[(size_t)p * 6], k);
 }
 
-__device__ __forceinline__ void tri_keys(const Cand& c, f3 v0, f3 v1, f3 v2, int32_t k[6]) {
+// Keys of one triangle's projection range on the node's axes.  The
+// reference reduces the three projections with kr_min / kr_max from vertex 0
+// and lets the result enter the node's range only when strictly beyond it
+// (transforms.cu:855-897).  For finite vertices (fin) the dot products are
+// never NaN and that is the integer min / max of the keys (up to the sign of
+// a zero).  Otherwise the compare-and-select chain is evaluated as written,
+// and a NaN result gives the neutral key, i.e. never enters the range.
+__device__ __forceinline__ void tri_keys(const Cand& c, f3 v0, f3 v1, f3 v2, bool fin, int32_t k[6]) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         const f3 b = f3{c.b[3 * a], c.b[3 * a + 1], c.b[3 * a + 2]};
-        const int32_t k0 = okey(dot3(v0, b)), k1 = okey(dot3(v1, b)), k2 = okey(dot3(v2, b));
-        k[a] = min(min(k0, k1), k2);
-        k[3 + a] = max(max(k0, k1), k2);
+        const float d0 = dot3(v0, b), d1 = dot3(v1, b), d2 = dot3(v2, b);
+        if (fin) {
+            const int32_t k0 = okey(d0), k1 = okey(d1), k2 = okey(d2);
+            k[a] = min(min(k0, k1), k2);
+            k[3 + a] = max(max(k0, k1), k2);
+        } else {
+            const float lo = kmin(kmin(d0, d1), d2), hi = kmax(kmax(d0, d1), d2);
+            k[a] = lo == lo ? okey(lo) : kKeyHi;
+            k[3 + a] = hi == hi ? okey(hi) : kKeyLo;
+        }
     }
+}
+
+// All coordinates finite (exponent field not all ones).
+__device__ __forceinline__ bool finite3(f3 v) {
+    const uint32_t m = 0x7F800000u;
+    return (__float_as_uint(v.x) & m) != m && (__float_as_uint(v.y) & m) != m && (__float_as_uint(v.z) & m) != m;
 }
 
 __device__ __forceinline__ void wave_keys(int32_t k[6]) {
@@ -1037,11 +1057,16 @@ __global__ __launch_bounds__(kRTile) void refit(const kr_bvh_node* __restrict__ 
     const uint32_t whi = min(wlo + kWave - 1, N - 1);
     const bool valid = pos < N;
     f3 v0 = f3{0, 0, 0}, v1 = v0, v2 = v0;
+    bool fin = true;
     if (valid) {
         const kr_uvec4 face = F[nodes[I + pos].u.primitivesOffset];
         v0 = ldv(V, face.x);
         v1 = ldv(V, face.y);
         v2 = ldv(V, face.z);
+        fin = finite3(v0) && finite3(v1) && finite3(v2);
+        // the tile-wide pass projects vertices, not triangles: a tile with a
+        // non-finite vertex keeps the per-triangle rule up to the root
+        if (!fin) R.overflow = 1;
         const uint32_t vid[3] = {face.x, face.y, face.z};
 #pragma unroll
         for (int j = 0; j < 3; ++j) {  // distinct vertex ids of the tile
@@ -1076,7 +1101,7 @@ __global__ __launch_bounds__(kRTile) void refit(const kr_bvh_node* __restrict__ 
                 if (holds(sp, wlo, whi)) { common = p; break; }
                 if (!c.degen) {
                     int32_t k[6];
-                    tri_keys(c, v0, v1, v2, k);
+                    tri_keys(c, v0, v1, v2, fin, k);
                     fold_tile(R, ws.range, lo, p, k);
                 }
                 p = pp;
@@ -1098,7 +1123,7 @@ __global__ __launch_bounds__(kRTile) void refit(const kr_bvh_node* __restrict__ 
                 if (!c.degen) {
                     int32_t k[6];
                     if (valid) {
-                        tri_keys(c, v0, v1, v2, k);
+                        tri_keys(c, v0, v1, v2, fin, k);
                     } else {
 #pragma unroll
                         for (int a = 0; a < 3; ++a) { k[a] = kKeyHi; k[3 + a] = kKeyLo; }
@@ -1200,7 +1225,7 @@ __global__ __launch_bounds__(kRTile) void refit(const kr_bvh_node* __restrict__ 
                 if (!c.degen) {
                     int32_t k[6];
                     if (valid) {
-                        tri_keys(c, v0, v1, v2, k);
+                        tri_keys(c, v0, v1, v2, fin, k);
                     } else {
 #pragma unroll
                         for (int a = 0; a < 3; ++a) { k[a] = kKeyHi; k[3 + a] = kKeyLo; }
@@ -1314,6 +1339,8 @@ __global__ void copy_ids(const kr_bvh_node* __restrict__ nodes, const kr_vec3* _
     }
 }
 
+__global__ void copy_status(const uint32_t* error, uint32_t* status) { *status = *error; }
+
 // ------------------------------------------------------------- host --
 
 size_t tree_carve(TreeWs* ws, void* base, size_t cap, uint32_t N) {
@@ -1411,6 +1438,25 @@ kr_error tree_launch(const kr_bvh_node* nodes, const uint32_t* parents, const kr
         hipLaunchKernelGGL(copy_ids, blocks(2 * N - 1, 256), dim3(256), 0, stream, nodes, V, F, ws, ids, 2 * N - 1);
     OBVH_HIP_TRY(hipStreamWaitEvent(stream, side->join, 0));
     OBVH_HIP_TRY(hipGetLastError());
+    return nullptr;
+}
+
+kr_error tree_status_launch(const void* workspace, size_t ws_bytes, uint32_t N, uint32_t* status,
+                            hipStream_t stream) {
+    TreeWs ws;
+    const size_t need = tree_carve(&ws, const_cast<void*>(workspace), ws_bytes, N);
+    if (workspace == nullptr || ws_bytes < need) return "obvh_obb_tree_status: workspace too small";
+    hipLaunchKernelGGL(copy_status, dim3(1), dim3(1), 0, stream, ws.error, status);
+    OBVH_HIP_TRY(hipGetLastError());
+    return nullptr;
+}
+
+const char* tree_status_string(uint32_t err) {
+    if (err & 2u) return "obvh_obb_tree: parent/child links disagree";
+    if (err & 1u) return "obvh_obb_tree: parent index out of range or cycle";
+    if (err & 4u) return "obvh_obb_tree: leaf face id out of range";
+    if (err & 8u) return "obvh_obb_tree: internal node never completed (malformed tree)";
+    if (err != 0u) return "obvh_obb_tree: unknown device error";
     return nullptr;
 }
 

@@ -45,4 +45,10 @@ kr_error tree_launch(const kr_bvh_node* nodes, const uint32_t* parents, const kr
                      const kr_uvec4* F, kr_obb3* obbs, kr_mat4* mats, uint32_t* ids, uint32_t N,
                      void* workspace, size_t ws_bytes, hipStream_t stream);
 
+// Copy the call's device error word (0 = success) into `status` (device or
+// pinned host memory) on `stream`, and its meaning.
+kr_error tree_status_launch(const void* workspace, size_t ws_bytes, uint32_t N, uint32_t* status,
+                            hipStream_t stream);
+const char* tree_status_string(uint32_t err);
+
 }  // namespace obvh

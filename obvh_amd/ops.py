@@ -50,6 +50,15 @@ def _stream(stream=None) -> C.c_void_p:
     return C.c_void_p(s.cuda_stream)
 
 
+def _on(stream):
+    """Context that makes `stream` torch's current stream, so that tensors
+    allocated inside belong to its caching-allocator pool: they are not
+    handed to another stream while this stream's kernels may still use
+    them."""
+    torch = _torch()
+    return torch.cuda.stream(stream if stream is not None else torch.cuda.current_stream())
+
+
 def _workspace(nbytes: int):
     torch = _torch()
     return torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device="cuda")
@@ -79,10 +88,11 @@ def dito(x, y, z, trace: bool = False, stream=None, workspace=None):
     for a in (x, y, z):
         if a.dtype != torch.float32 or not a.is_cuda or not a.is_contiguous():
             raise ValueError("points must be contiguous float32 CUDA tensors")
-    out = torch.empty(15, dtype=torch.float32, device="cuda")
-    tr = torch.zeros(TRACE_DTYPE.itemsize, dtype=torch.uint8, device="cuda") if trace else None
-    need = lib().obvh_dito_workspace_size(n)
-    ws = workspace if workspace is not None else _workspace(need)
+    with _on(stream):
+        out = torch.empty(15, dtype=torch.float32, device="cuda")
+        tr = torch.zeros(TRACE_DTYPE.itemsize, dtype=torch.uint8, device="cuda") if trace else None
+        need = lib().obvh_dito_workspace_size(n)
+        ws = workspace if workspace is not None else _workspace(need)
     check(lib().obvh_dito_soa(_ptr(x), _ptr(y), _ptr(z), n, _ptr(out), _ptr(tr), _ptr(ws),
                               ws.numel(), _stream(stream)))
     if trace:
@@ -91,17 +101,25 @@ def dito(x, y, z, trace: bool = False, stream=None, workspace=None):
 
 
 def dito_batch(x, y, z, offsets, tile_points: int = 0, workers: int = 0, trace: bool = False,
-               stream=None, workspace=None):
-    """Batched DiTO.  offsets: int64 CUDA tensor with nsets + 1 entries."""
+               stream=None, workspace=None, host_offsets=None):
+    """Batched DiTO.  offsets: int64 CUDA tensor with nsets + 1 entries;
+    host_offsets (optional): the same values on the host, validated here.
+    Without them a set with decreasing offsets or offsets past the points
+    gets a NaN box (trace degenerate = 4) on the device."""
     torch = _torch()
     nsets = int(offsets.numel()) - 1
     total = int(x.numel())
     if offsets.dtype != torch.int64 or not offsets.is_cuda:
         raise ValueError("offsets must be an int64 CUDA tensor")
-    out = torch.empty((max(nsets, 0), 15), dtype=torch.float32, device="cuda")
-    tr = torch.zeros(max(nsets, 1) * TRACE_DTYPE.itemsize, dtype=torch.uint8, device="cuda") if trace else None
-    need = lib().obvh_dito_batch_workspace_size(max(nsets, 1), total, tile_points)
-    ws = workspace if workspace is not None else _workspace(need)
+    if host_offsets is not None:
+        ho = np.asarray(host_offsets, dtype=np.int64)
+        if ho.shape != (nsets + 1,) or (nsets > 0 and (np.any(np.diff(ho) < 0) or ho[0] < 0 or ho[-1] > total)):
+            raise ValueError("offsets must be nsets + 1 non-decreasing values within [0, total points]")
+    with _on(stream):
+        out = torch.empty((max(nsets, 0), 15), dtype=torch.float32, device="cuda")
+        tr = torch.zeros(max(nsets, 1) * TRACE_DTYPE.itemsize, dtype=torch.uint8, device="cuda") if trace else None
+        need = lib().obvh_dito_batch_workspace_size(max(nsets, 1), total, tile_points)
+        ws = workspace if workspace is not None else _workspace(need)
     if ws.numel() < need:
         raise ValueError(f"workspace needs {need} bytes")
     check(lib().obvh_dito_batch(_ptr(x), _ptr(y), _ptr(z), _ptr(offsets), nsets, total, _ptr(out), _ptr(tr),
@@ -126,12 +144,13 @@ def lbvh_build(V, F, stream=None):
     torch = _torch()
     nf = int(F.shape[0])
     total = 2 * nf - 1
-    nodes = torch.empty(total * NODE_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
-    parents = torch.empty(total, dtype=torch.int32, device="cuda")
-    counts = torch.empty(total, dtype=torch.int32, device="cuda")
-    costs = torch.empty(total, dtype=torch.float32, device="cuda")
-    mortons = torch.empty(nf, dtype=torch.int32, device="cuda")
-    ws = _workspace(lib().obvh_lbvh_workspace_size(nf))
+    with _on(stream):
+        nodes = torch.empty(total * NODE_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+        parents = torch.empty(total, dtype=torch.int32, device="cuda")
+        counts = torch.empty(total, dtype=torch.int32, device="cuda")
+        costs = torch.empty(total, dtype=torch.float32, device="cuda")
+        mortons = torch.empty(nf, dtype=torch.int32, device="cuda")
+        ws = _workspace(lib().obvh_lbvh_workspace_size(nf))
     check(lib().obvh_lbvh_build(_ptr(V), _ptr(F), nf, _ptr(nodes), _ptr(parents), _ptr(counts), _ptr(costs),
                                 _ptr(mortons), _ptr(ws), ws.numel(), _stream(stream)))
     return {"nodes": nodes, "parents": parents, "counts": counts, "costs": costs, "mortons": mortons}
@@ -144,12 +163,13 @@ def atrbvh_build(V, F, treelet_size: int = 7, iterations: int = 2, ci: float = 1
     torch = _torch()
     nf = int(F.shape[0])
     total = 2 * nf - 1
-    nodes = torch.empty(total * NODE_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
-    parents = torch.empty(total, dtype=torch.int32, device="cuda")
-    counts = torch.empty(total, dtype=torch.int32, device="cuda")
-    costs = torch.empty(total, dtype=torch.float32, device="cuda")
-    need = lib().obvh_atrbvh_workspace_size(nf)
-    ws = workspace if workspace is not None else _workspace(need)
+    with _on(stream):
+        nodes = torch.empty(total * NODE_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+        parents = torch.empty(total, dtype=torch.int32, device="cuda")
+        counts = torch.empty(total, dtype=torch.int32, device="cuda")
+        costs = torch.empty(total, dtype=torch.float32, device="cuda")
+        need = lib().obvh_atrbvh_workspace_size(nf)
+        ws = workspace if workspace is not None else _workspace(need)
     if ws.numel() < need:
         raise ValueError(f"workspace needs {need} bytes")
     check(lib().obvh_atrbvh_build(_ptr(V), _ptr(F), nf, int(treelet_size), int(iterations), C.c_float(ci),
@@ -164,24 +184,41 @@ def nodes_to_numpy(nodes) -> np.ndarray:
 
 # ----------------------------------------------------------- transform --
 
-def obb_tree(nodes, parents, V, F, with_ids: bool = False, stream=None, workspace=None, out=None):
-    """Stream-ordered transform.  Returns (obbs (2N-1,15), mats (2N-1,16)[, ids (2N-1,14)])."""
+def obb_tree(nodes, parents, V, F, with_ids: bool = False, stream=None, workspace=None, out=None,
+             status=None):
+    """Stream-ordered transform.  Returns (obbs (2N-1,15), mats (2N-1,16)[, ids (2N-1,14)]).
+
+    status: optional int32 tensor of one element (device or pinned host)
+    that receives the call's device error word on the same stream (0 =
+    success; tree_status_message() explains other values).  A malformed tree
+    is only detected on the device."""
     torch = _torch()
     total = int(parents.numel())
     N = (total + 1) // 2
-    if out is None:
-        obbs = torch.empty((total, 15), dtype=torch.float32, device="cuda")
-        mats = torch.empty((total, 16), dtype=torch.float32, device="cuda")
-    else:
-        obbs, mats = out
-    ids = torch.empty((total, 14), dtype=torch.int32, device="cuda") if with_ids else None
-    need = lib().obvh_obb_tree_workspace_size(N, N - 1)
-    ws = workspace if workspace is not None else _workspace(need)
+    with _on(stream):
+        if out is None:
+            obbs = torch.empty((total, 15), dtype=torch.float32, device="cuda")
+            mats = torch.empty((total, 16), dtype=torch.float32, device="cuda")
+        else:
+            obbs, mats = out
+        ids = torch.empty((total, 14), dtype=torch.int32, device="cuda") if with_ids else None
+        need = lib().obvh_obb_tree_workspace_size(N, N - 1)
+        ws = workspace if workspace is not None else _workspace(need)
     if ws.numel() < need:
         raise ValueError(f"workspace needs {need} bytes")
     check(lib().obvh_obb_tree_async(_ptr(nodes), _ptr(parents), _ptr(V), _ptr(F), _ptr(obbs), _ptr(mats),
                                     _ptr(ids), N, N - 1, _ptr(ws), ws.numel(), _stream(stream)))
+    if status is not None:
+        if status.dtype != torch.int32 or status.numel() < 1:
+            raise ValueError("status must be an int32 tensor with one element")
+        check(lib().obvh_obb_tree_status_async(_ptr(ws), ws.numel(), N, _ptr(status), _stream(stream)))
     return (obbs, mats, ids) if with_ids else (obbs, mats)
+
+
+def tree_status_message(word: int):
+    """Error message of a nonzero transform status word (None for 0)."""
+    m = lib().obvh_obb_tree_status_string(int(word) & 0xFFFFFFFF)
+    return m.decode() if m else None
 
 
 def bvh_obb_tree(nodes, parents, V, F, obbs, mats, primitive_count: int, internal_count: int,
@@ -211,12 +248,13 @@ def collapse(nodes, parents, counts, ci: float = 1.2, ct: float = 1.0, stream=No
     torch = _torch()
     total = int(parents.numel())
     N = (total + 1) // 2
-    out = torch.empty(total * NODE_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
-    cnt = counts.clone()
-    costs = torch.empty(total, dtype=torch.float32, device="cuda")
-    prims = torch.full((N,), -1, dtype=torch.int32, device="cuda")
-    need = lib().obvh_collapse_workspace_size(N)
-    ws = workspace if workspace is not None else _workspace(need)
+    with _on(stream):
+        out = torch.empty(total * NODE_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+        cnt = counts.clone()
+        costs = torch.empty(total, dtype=torch.float32, device="cuda")
+        prims = torch.full((N,), -1, dtype=torch.int32, device="cuda")
+        need = lib().obvh_collapse_workspace_size(N)
+        ws = workspace if workspace is not None else _workspace(need)
     if ws.numel() < need:
         raise ValueError(f"workspace needs {need} bytes")
     check(lib().obvh_collapse_async(_ptr(nodes), _ptr(parents), _ptr(out), _ptr(cnt), _ptr(prims), _ptr(costs),
@@ -262,9 +300,10 @@ def soa_pack(nodes, mats, stream=None):
     torch = _torch()
     total = nodes.numel() // NODE_DTYPE.itemsize
     internal = (total - 1) // 2
-    L = torch.empty((max(internal, 1), 12), dtype=torch.float32, device="cuda")
-    R = torch.empty((max(internal, 1), 12), dtype=torch.float32, device="cuda")
-    meta = torch.empty((total, 4), dtype=torch.int32, device="cuda")
+    with _on(stream):
+        L = torch.empty((max(internal, 1), 12), dtype=torch.float32, device="cuda")
+        R = torch.empty((max(internal, 1), 12), dtype=torch.float32, device="cuda")
+        meta = torch.empty((total, 4), dtype=torch.int32, device="cuda")
     check(lib().obvh_soa_pack_async(_ptr(nodes), _ptr(mats), total, internal, _ptr(L), _ptr(R), _ptr(meta),
                                     _stream(stream)))
     return {"left_T": L[:internal], "right_T": R[:internal], "meta": meta}
@@ -275,13 +314,14 @@ def soa_intersect(packed, V, F, rays, primitives=None, stream=None, workspace=No
     RAY_DTYPE numpy array) against a packed OBVH.  Returns a uint8 tensor of
     kr_intersection records (view the host copy with ISECT_DTYPE)."""
     torch = _torch()
-    if isinstance(rays, np.ndarray):
-        rays = torch.from_numpy(np.ascontiguousarray(rays, dtype=RAY_DTYPE).view(np.uint8)).cuda()
-    n = rays.numel() // RAY_DTYPE.itemsize
-    if out is None:
-        out = torch.empty(n * ISECT_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
-    need = lib().obvh_soa_intersect_workspace_size(n)
-    ws = workspace if workspace is not None else _workspace(need)
+    with _on(stream):
+        if isinstance(rays, np.ndarray):
+            rays = torch.from_numpy(np.ascontiguousarray(rays, dtype=RAY_DTYPE).view(np.uint8)).cuda()
+        n = rays.numel() // RAY_DTYPE.itemsize
+        if out is None:
+            out = torch.empty(n * ISECT_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+        need = lib().obvh_soa_intersect_workspace_size(n)
+        ws = workspace if workspace is not None else _workspace(need)
     if ws.numel() < need:
         raise ValueError(f"workspace needs {need} bytes")
     nf = F.numel() // 4
@@ -299,8 +339,9 @@ def records_pack(nodes, mats, V, F, primitives=None, stream=None):
     torch = _torch()
     total = nodes.numel() // NODE_DTYPE.itemsize
     N = (total + 1) // 2
-    rec = torch.zeros(max(N - 1, 1) * 128, dtype=torch.uint8, device="cuda")
-    tri = torch.zeros(2 * N * 48, dtype=torch.uint8, device="cuda")
+    with _on(stream):
+        rec = torch.zeros(max(N - 1, 1) * 128, dtype=torch.uint8, device="cuda")
+        tri = torch.zeros(2 * N * 48, dtype=torch.uint8, device="cuda")
     check(lib().obvh_records_pack_async(_ptr(nodes), _ptr(mats), _ptr(primitives), _ptr(V), V.numel() // 3,
                                         _ptr(F), N, _ptr(rec), _ptr(tri), _stream(stream)))
     return {"nodes": rec, "tris": tri, "face_count": N}
@@ -309,13 +350,14 @@ def records_pack(nodes, mats, V, F, primitives=None, stream=None):
 def records_intersect(records, rays, stream=None, workspace=None, out=None):
     """Closest hits against the record layout; same output as soa_intersect."""
     torch = _torch()
-    if isinstance(rays, np.ndarray):
-        rays = torch.from_numpy(np.ascontiguousarray(rays, dtype=RAY_DTYPE).view(np.uint8)).cuda()
-    n = rays.numel() // RAY_DTYPE.itemsize
-    if out is None:
-        out = torch.empty(n * ISECT_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
-    need = lib().obvh_soa_intersect_workspace_size(n)
-    ws = workspace if workspace is not None else _workspace(need)
+    with _on(stream):
+        if isinstance(rays, np.ndarray):
+            rays = torch.from_numpy(np.ascontiguousarray(rays, dtype=RAY_DTYPE).view(np.uint8)).cuda()
+        n = rays.numel() // RAY_DTYPE.itemsize
+        if out is None:
+            out = torch.empty(n * ISECT_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+        need = lib().obvh_soa_intersect_workspace_size(n)
+        ws = workspace if workspace is not None else _workspace(need)
     if ws.numel() < need:
         raise ValueError(f"workspace needs {need} bytes")
     check(lib().obvh_records_intersect_async(_ptr(records["nodes"]), _ptr(records["tris"]), records["face_count"],

@@ -46,6 +46,7 @@ class Request:
     faces: object = None      # pinned torch tensor (F, 4) int32
     done: float = math.nan
     result: object = None
+    error: str | None = None  # set instead of result when the request failed
     nbytes: int = 0
 
 
@@ -179,14 +180,23 @@ class Server:
             V = r.vertices.to("cuda", non_blocking=True)
             F = r.faces.to("cuda", non_blocking=True)
             t = ops.lbvh_build(V, F, stream=stream)
-            obbs, mats = ops.obb_tree(t["nodes"], t["parents"], V, F, stream=stream)
+            status = torch.full((1,), -1, dtype=torch.int32, pin_memory=True)
+            obbs, mats = ops.obb_tree(t["nodes"], t["parents"], V, F, stream=stream, status=status)
             root = torch.empty(15, dtype=torch.float32, pin_memory=True)
             root.copy_(obbs[0], non_blocking=True)
             ev = torch.cuda.Event()
             ev.record(stream)
         keep = (V, F, t)
 
-        def finish(now, r=r, root=root, obbs=obbs, mats=mats, keep=keep, stream=stream):
+        def finish(now, r=r, root=root, obbs=obbs, mats=mats, keep=keep, stream=stream, status=status):
+            word = int(status.item())
+            if word != 0:  # the device found a malformed tree: fail the request
+                r.error = ops.tree_status_message(word)
+                r.result = None
+                r.done = now
+                self.in_use -= r.nbytes
+                self.finished.append(r)
+                return
             r.result = {"root": root.numpy().copy()}
             if self.keep_trees:
                 r.result.update({"obbs": obbs, "mats": mats, "nodes": keep[2]["nodes"], "stream": stream})

@@ -133,7 +133,8 @@ def host_threads() -> int:
 def obb_tree(nodes, parents, vertices, faces, with_ids: bool = False, with_ranges: bool = False,
              with_axes: bool = False, threads: int = 1):
     """The transform method.  Returns obbs (2N-1, 15), mats (2N-1, 16)
-    [, ids (2N-1, 14)] [, ranges (N-1, 6)] [, candidate axes (N-1, 10)].
+    [, ids (2N-1, 14)] [, ranges (N-1, 6)] [, candidate axes (N-1, 11):
+    b0, b1, b2, degenerate flag, winning candidate id (-1: AABB axes)].
     threads > 1 runs it node-parallel on host threads (same outputs)."""
     V = _f32(vertices).reshape(-1, 3)
     F = np.ascontiguousarray(faces, dtype=np.uint32).reshape(-1, 4)
@@ -145,7 +146,7 @@ def obb_tree(nodes, parents, vertices, faces, with_ids: bool = False, with_range
     mats = np.zeros((total, 16), dtype=np.float32)
     ids = np.zeros((total, 14), dtype=np.uint32) if with_ids else None
     rr = np.zeros((max(N - 1, 1), 6), dtype=np.float32) if with_ranges else None
-    ca = np.zeros((max(N - 1, 1), 10), dtype=np.float32) if with_axes else None
+    ca = np.zeros((max(N - 1, 1), 11), dtype=np.float32) if with_axes else None
     rc = lib().oracle_obb_tree_mt(
         _p(nodes), _p(parents), _p(V), _p(F), N, _p(obbs), _p(mats),
         _p(ids) if ids is not None else None, _p(rr) if rr is not None else None,

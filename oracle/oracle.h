@@ -79,8 +79,10 @@ int oracle_parallel_for(uint32_t n, uint32_t chunk, int threads, opar_fn fn, voi
  * outputs obbs/mats for every node.  kdop_ids (nullable): 14 ids per node
  * [min0..min6, max0..max6].  refit_ranges (nullable): 6 floats per node
  * (min b0..b2, max b0..b2 after refit; internal nodes only).  cand_axes
- * (nullable): 10 floats per internal node, the candidate axes b0, b1, b2
- * and 1.0 if the node is degenerate (its box is its AABB).
+ * (nullable): 11 floats per internal node, the candidate axes b0, b1, b2,
+ * 1.0 if the node is degenerate (its box is its AABB) and the winning
+ * candidate id (0..2 base triangle, 3..11 upper, 12..20 lower tetrahedron,
+ * -1 the AABB's axes).
  * Returns 0 on success, -1 on a malformed tree. */
 int oracle_obb_tree(const onode* nodes, const uint32_t* parents,
                     const float* vertices, const uint32_t* faces /* uvec4 */,

@@ -194,6 +194,8 @@ typedef struct {
     ov3 b0, b1, b2;
     ov3 rmin, rmax;   /* candidate ranges over the 14 points (min_proj/max_proj[3..5]) */
     int degenerate;   /* 1: identity axes, the node's result is its AABB */
+    int best_id;      /* winning candidate (DiTO's numbering: 0..2 base triangle,
+                         3..11 upper, 12..20 lower tetrahedron), -1: the AABB's axes */
 } candidate;
 
 static void ext_proj(const ov3* P, ov3 d, float* mn, float* mx) { /* :418-435 */
@@ -209,7 +211,7 @@ static void ext_proj(const ov3* P, ov3 d, float* mn, float* mx) { /* :418-435 */
 
 /* transforms.cu:484-544 */
 static void from_normal_edges(const ov3* P, ov3 n, ov3 e0, ov3 e1, ov3 e2, candidate* c,
-                              float* best) {
+                              float* best, int id0) {
     ov3 e[3] = {e0, e1, e2};
     ov3 m[3] = {vcross(e0, n), vcross(e1, n), vcross(e2, n)};
     ov3 dmin, dmax;
@@ -226,6 +228,7 @@ static void from_normal_edges(const ov3* P, ov3 n, ov3 e0, ov3 e1, ov3 e2, candi
             c->rmax = dmax;
             c->rmin = dmin;
             c->b0 = e[k]; c->b1 = n; c->b2 = m[k];
+            c->best_id = id0 + k;
         }
     }
 }
@@ -239,6 +242,7 @@ static void candidate_search(const float* V, const kdop* k, candidate* c) {
     c->rmin = v3(k->mn[0], k->mn[1], k->mn[2]);   /* :706-707 */
     c->rmax = v3(k->mx[0], k->mx[1], k->mx[2]);
     c->degenerate = 0;
+    c->best_id = -1;
 
     /* Base triangle, :439-455,546-579. */
     int at = 0;
@@ -261,7 +265,7 @@ static void candidate_search(const float* V, const kdop* k, candidate* c) {
     ov3 e1 = vnormalize(vsub(p1, p2));
     ov3 e2 = vnormalize(vsub(p2, p0));
     ov3 n = vnormalize(vcross(e1, e0));
-    from_normal_edges(P, n, e0, e1, e2, c, &best);
+    from_normal_edges(P, n, e0, e1, e2, c, &best, 0);
 
     /* Ditetrahedron, :582-674. */
     float pr = vdot(P[0], n);
@@ -284,9 +288,9 @@ static void candidate_search(const float* V, const kdop* k, candidate* c) {
         ov3 n0 = vnormalize(vcross(f1, e0));
         ov3 n1 = vnormalize(vcross(f2, e1));
         ov3 n2 = vnormalize(vcross(f0, e2));
-        from_normal_edges(P, n0, e0, f1, f0, c, &best);
-        from_normal_edges(P, n1, e1, f2, f1, c, &best);
-        from_normal_edges(P, n2, e2, f0, f2, c, &best);
+        from_normal_edges(P, n0, e0, f1, f0, c, &best, 3);
+        from_normal_edges(P, n1, e1, f2, f1, c, &best, 6);
+        from_normal_edges(P, n2, e2, f0, f2, c, &best, 9);
     }
     if (q1v) {
         ov3 q1 = P[ilo];
@@ -296,9 +300,9 @@ static void candidate_search(const float* V, const kdop* k, candidate* c) {
         ov3 m0 = vnormalize(vcross(g1, e0));
         ov3 m1 = vnormalize(vcross(g2, e1));
         ov3 m2 = vnormalize(vcross(g0, e2));
-        from_normal_edges(P, m0, e0, g1, g0, c, &best);
-        from_normal_edges(P, m1, e1, g2, g1, c, &best);
-        from_normal_edges(P, m2, e2, g0, g2, c, &best);
+        from_normal_edges(P, m0, e0, g1, g0, c, &best, 12);
+        from_normal_edges(P, m1, e1, g2, g1, c, &best, 15);
+        from_normal_edges(P, m2, e2, g0, g2, c, &best, 18);
     }
 }
 
@@ -386,7 +390,15 @@ static void candidates_one(void* p, uint32_t v, int tid) {
 }
 
 /* Refit: exact ranges over every subtree vertex (transforms.cu:836-900),
- * computed per node over its own subtree; then the node's box (K4). */
+ * computed per node over its own subtree; then the node's box (K4).
+ * As in the reference kernel, each triangle first reduces its three
+ * projections with kr_min / kr_max from vertex 0 (:855-885), and the result
+ * enters the node's range only when strictly beyond it (:887-897, the
+ * compare before the float CAS loop of atomics.h:9-31).  For finite input
+ * that is the plain min / max over the vertices (up to the sign of a zero);
+ * with a NaN projection it fixes which values survive: a NaN in the running
+ * range of a triangle is replaced by the next value, a NaN triangle result
+ * never enters the node's range. */
 static void refit_one(void* p, uint32_t v, int tid) {
     const tree_ctx* t = (const tree_ctx*)p;
     const onode* nodes = t->nodes;
@@ -401,21 +413,31 @@ static void refit_one(void* p, uint32_t v, int tid) {
             uint32_t u = stack[--sp];
             if (u < I) { stack[sp++] = nodes[u].a; stack[sp++] = nodes[u].b; continue; }
             const uint32_t* f = &t->faces[4 * nodes[u].a];
-            for (int j = 0; j < 3; ++j) {
-                ov3 q = vert(t->V, f[j]);
+            ov3 q = vert(t->V, f[0]);
+            ov3 tmin = v3(vdot(q, c->b0), vdot(q, c->b1), vdot(q, c->b2));
+            ov3 tmax = tmin;
+            for (int j = 1; j < 3; ++j) {
+                q = vert(t->V, f[j]);
                 float d0 = vdot(q, c->b0), d1 = vdot(q, c->b1), d2 = vdot(q, c->b2);
-                rmin.x = kmin(rmin.x, d0); rmax.x = kmax(rmax.x, d0);
-                rmin.y = kmin(rmin.y, d1); rmax.y = kmax(rmax.y, d1);
-                rmin.z = kmin(rmin.z, d2); rmax.z = kmax(rmax.z, d2);
+                tmin.x = kmin(tmin.x, d0); tmax.x = kmax(tmax.x, d0);
+                tmin.y = kmin(tmin.y, d1); tmax.y = kmax(tmax.y, d1);
+                tmin.z = kmin(tmin.z, d2); tmax.z = kmax(tmax.z, d2);
             }
+            if (tmin.x < rmin.x) rmin.x = tmin.x;
+            if (tmin.y < rmin.y) rmin.y = tmin.y;
+            if (tmin.z < rmin.z) rmin.z = tmin.z;
+            if (tmax.x > rmax.x) rmax.x = tmax.x;
+            if (tmax.y > rmax.y) rmax.y = tmax.y;
+            if (tmax.z > rmax.z) rmax.z = tmax.z;
         }
     }
     if (t->cand_axes) {
-        float* ca = &t->cand_axes[10 * v];
+        float* ca = &t->cand_axes[11 * v];
         ca[0] = c->b0.x; ca[1] = c->b0.y; ca[2] = c->b0.z;
         ca[3] = c->b1.x; ca[4] = c->b1.y; ca[5] = c->b1.z;
         ca[6] = c->b2.x; ca[7] = c->b2.y; ca[8] = c->b2.z;
         ca[9] = (float)c->degenerate;
+        ca[10] = (float)c->best_id;
     }
     if (t->refit_ranges) {
         float* rr = &t->refit_ranges[6 * v];

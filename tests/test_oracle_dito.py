@@ -164,3 +164,80 @@ def test_threaded_batch_equals_sequential(oracle_lib, threads):
     outn, trn = oracle_lib.points_obb_batch(pts, off, threads=threads)
     assert np.array_equal(out1.view(np.uint32), outn.view(np.uint32))
     assert tr1.tobytes() == trn.tobytes()
+
+
+# ---------------------------------------------------------------- non-finite --
+# dito.cpp:116-173 and :193-206 start every running extreme at point 0 and
+# compare strictly, so a NaN never wins a compare and a NaN at point 0 is
+# never replaced.  These fix the oracle's answer on non-finite input without
+# its code: a point that is NaN in every coordinate behaves exactly like a
+# copy of point 0 (which also never wins a strict compare), and a NaN point 0
+# leaves every slab at (NaN, index 0).
+
+def _bits_equal(a, b):
+    a = np.asarray(a, np.float32)
+    b = np.asarray(b, np.float32)
+    return np.array_equal(np.isnan(a), np.isnan(b)) and np.array_equal(a[~np.isnan(a)], b[~np.isnan(b)])
+
+
+@pytest.mark.parametrize("j", [1, 7, 500, 999])
+def test_nan_point_mid_set_acts_like_a_copy_of_point_0(oracle_lib, j):
+    p = synth.gaussian_points(1000, seed=41)
+    q = p.copy()
+    q[j] = np.nan
+    r = p.copy()
+    r[j] = p[0]
+    got, gtr = oracle_lib.points_obb(q, trace=True)
+    exp, etr = oracle_lib.points_obb(r, trace=True)
+    assert _bits_equal(got, exp)
+    for f in ("slab_min_idx", "slab_max_idx", "pair_k", "p2_sel", "q0_sel", "q1_sel", "best", "result_aabb"):
+        assert np.array_equal(gtr[f], etr[f]), f
+    assert not np.isnan(got).any()
+
+
+@pytest.mark.parametrize("n", [1, 10, 1000])
+def test_nan_point_0_sticks(oracle_lib, n):
+    p = synth.gaussian_points(n, seed=42)
+    p[0] = np.nan
+    o, tr = oracle_lib.points_obb(p, trace=True)
+    mid, ax, ext = obb_parts(o)
+    assert np.all(tr["slab_min_idx"] == 0) and np.all(tr["slab_max_idx"] == 0)
+    # every comparison against NaN fails: no candidate beats the NaN AABB
+    # quality, the box is the AABB branch with NaN mid and extents
+    assert tr["best"] == -1 and tr["result_aabb"] == 1
+    assert np.isnan(mid).all() and np.isnan(ext).all()
+    assert np.array_equal(ax, np.eye(3))
+
+
+def test_nan_x_of_point_0_sticks_on_x_slabs_only(oracle_lib):
+    p = synth.gaussian_points(500, seed=43)
+    p[0, 0] = np.nan
+    _, tr = oracle_lib.points_obb(p, trace=True)
+    # slabs 0 and 3..6 contain x; slabs 1 (y) and 2 (z) are the plain
+    # first-index argmin / argmax over all points
+    for k in (0, 3, 4, 5, 6):
+        assert tr["slab_min_idx"][k] == 0 and tr["slab_max_idx"][k] == 0
+    assert tr["slab_min_idx"][1] == int(np.argmin(p[:, 1])) and tr["slab_max_idx"][1] == int(np.argmax(p[:, 1]))
+    assert tr["slab_min_idx"][2] == int(np.argmin(p[:, 2])) and tr["slab_max_idx"][2] == int(np.argmax(p[:, 2]))
+
+
+def test_infinite_coordinate_is_the_extreme(oracle_lib):
+    p = synth.gaussian_points(300, seed=44)
+    p[17, 0] = np.inf
+    p[40, 1] = -np.inf
+    _, tr = oracle_lib.points_obb(p, trace=True)
+    assert tr["slab_max_idx"][0] == 17 and tr["slab_min_idx"][1] == 40
+    # point 17 projects to +inf on slabs 3..6; point 40 to -inf on x+y+z and
+    # x+y-z, to +inf on x-y+z and x-y-z, where it ties with point 17 and the
+    # strict compare keeps the first (17)
+    for k in (3, 4, 5, 6):
+        assert tr["slab_max_idx"][k] == 17, k
+    assert tr["slab_min_idx"][3] == 40 and tr["slab_min_idx"][4] == 40
+    assert tr["slab_min_idx"][5] not in (17, 40) and tr["slab_min_idx"][6] not in (17, 40)
+
+
+def test_all_nan_set(oracle_lib):
+    p = np.full((64, 3), np.nan, np.float32)
+    o, tr = oracle_lib.points_obb(p, trace=True)
+    assert np.all(tr["slab_min_idx"] == 0) and np.all(tr["slab_max_idx"] == 0)
+    assert np.isnan(o[:3]).all() and np.isnan(o[12:]).all()

@@ -351,3 +351,244 @@ def test_transform_against_dito_of_each_subtree(oracle_lib, kind):
     assert abs(np.mean(rel)) < 0.01
     if kind == "hair":
         assert n_line > 0  # thin tubes do reach the line case
+
+
+# --------------------------------------------- candidate choice, node by node --
+
+_INTERLEAVE = [0, 7, 1, 8, 2, 9, 3, 10, 4, 11, 5, 12, 6, 13]
+
+
+def _candidate_axes64(P, tr):
+    """The 21 candidate axis triples of DiTO (dito.cpp:309-418) in float64,
+    built from the discrete choices of a DiTO trace (furthest pair, apex of
+    the base triangle, the two tetrahedron apexes).  None for a candidate
+    whose apex is not valid."""
+    P = P.astype(np.float64)
+    nz = lambda u: u / np.linalg.norm(u)
+    k = int(tr["pair_k"])
+    p0, p1, p2 = P[int(tr["slab_min_idx"][k])], P[int(tr["slab_max_idx"][k])], P[int(tr["p2_sel"])]
+    e0, e1, e2 = nz(p0 - p1), nz(p1 - p2), nz(p2 - p0)
+    n = nz(np.cross(e1, e0))
+    out = [(e, n, np.cross(e, n)) for e in (e0, e1, e2)]
+    for valid, qi in ((tr["q0_valid"], tr["q0_sel"]), (tr["q1_valid"], tr["q1_sel"])):
+        if not valid:
+            out += [None] * 9
+            continue
+        q = P[int(qi)]
+        f0, f1, f2 = nz(q - p0), nz(q - p1), nz(q - p2)
+        for nn, es in ((nz(np.cross(f1, e0)), (e0, f1, f0)), (nz(np.cross(f2, e1)), (e1, f2, f1)),
+                       (nz(np.cross(f0, e2)), (e2, f0, f2))):
+            out += [(e, nn, np.cross(e, nn)) for e in es]
+    return out
+
+
+def _quality64(P, axes):
+    d = P.astype(np.float64) @ np.array(axes).T
+    l = d.max(0) - d.min(0)
+    return l[0] * l[1] + l[0] * l[2] + l[1] * l[2]
+
+
+@pytest.mark.parametrize("kind", ["ico", "hair"])
+def test_candidate_choice_equals_dito_of_the_14_points(oracle_lib, kind):
+    """Node by node: the transform's candidate search (transforms.cu:677-756)
+    is DiTO's search (dito.cpp:347-418) over the node's 14 k-DOP points in
+    interleaved order, which kr_points_obb computes when given exactly those
+    14 points (n <= 14 searches all points in input order, dito.cpp:519-520).
+    The two differ only in how they normalise (reciprocal multiply against
+    divide, one rounding apart), so the winning candidate must be the same
+    one for all but near-tied nodes (>= 99.9 %), with the same axes to a
+    few ulp; a transform node that falls back to its AABB must be one DiTO
+    finds degenerate or without a valid apex; and, in float64, the winner's
+    quality must be the minimum over the valid candidates (or the AABB's,
+    when no candidate won) to within rounding.  Nodes where two distinct
+    points tie on a slab are skipped (DiTO and the k-DOP break such ties in
+    different point orders and then start from different base triangles).
+    A wrong choice rule (last instead of first, max instead of min, a
+    dropped apex) moves most nodes away from DiTO's choice and fails both
+    bounds."""
+    V, F = (synth.icosphere(14, seed=5) if kind == "ico" else synth.hairball(tubes=120, seed=6))
+    nodes, parents, *_ = oracle_lib.lbvh_build(V, F)
+    obbs, mats, ids, ca = oracle_lib.obb_tree(nodes, parents, V, F, with_ids=True, with_axes=True)
+    I = len(F) - 1
+    same = compared = slab_ties = untied = threshold = 0
+    # float32 rounding of a box's quality: ~1e-6 relative on the icosphere;
+    # on the tube walls sliver triangles carry normals with large rounding
+    # errors and float32 / float64 qualities differ by up to ~1e-3
+    tie_tol = 1e-5 if kind == "ico" else 1e-3
+    mismatches = []
+    excess = []
+    for v in range(I):
+        P = V[ids[v, _INTERLEAVE]]
+        ref, tr = oracle_lib.dito14(P, trace=True)
+        degen, best = int(ca[v, 9]), int(ca[v, 10])
+        dito_degen = int(tr["degenerate"]) in (1, 2) or (tr["q0_valid"] == 0 and tr["q1_valid"] == 0)
+        if degen or dito_degen:
+            # both must fall back to the AABB; the 1e-6 thresholds on squared
+            # distances (transforms.cu:554,613) are near-ties on the thin
+            # tubes, where the two normalisations can land on either side
+            compared += 1
+            if degen and dito_degen:
+                same += 1
+            else:
+                mismatches.append((v, -2 if degen else best, int(tr["best"]), "threshold"))
+                threshold += 1
+            continue
+        # DiTO's own slab pass over the 14 points must land on the same
+        # points as the k-DOP (it does unless two distinct points tie on a
+        # slab, where DiTO takes the first in interleaved order and the k-DOP
+        # the first in DFS order): otherwise the base triangles differ
+        mi, ma = np.asarray(tr["slab_min_idx"]), np.asarray(tr["slab_max_idx"])
+        if not (np.array_equal(P[mi], P[0::2]) and np.array_equal(P[ma], P[1::2])):
+            slab_ties += 1
+            continue
+        compared += 1
+        agree = best == int(tr["best"])
+        if agree and best >= 0 and int(tr["result_aabb"]) == 0:
+            # the same axes, signs and order, to rounding (a different base
+            # triangle can reuse a candidate number with other axes)
+            cos = np.sum(ca[v, :9].reshape(3, 3).astype(np.float64) * ref[3:12].reshape(3, 3), axis=1)
+            agree = bool(np.all(cos > 1 - 1e-6))
+        if agree:
+            same += 1
+        else:
+            # a different winner is acceptable only as a tie: in float64 the
+            # two winners' boxes over the 14 points have the same quality to
+            # float32 rounding (symmetric point sets, e.g. a few points of a
+            # tube wall, tie exactly in real arithmetic)
+            ax_t = ca[v, :9].reshape(3, 3) if best >= 0 else np.eye(3)
+            ax_d = ref[3:12].reshape(3, 3) if int(tr["best"]) >= 0 and int(tr["result_aabb"]) == 0 else np.eye(3)
+            qt, qd = _quality64(P, ax_t.astype(np.float64)), _quality64(P, ax_d.astype(np.float64))
+            tie = abs(qt - qd) <= tie_tol * max(qd, 1e-30)
+            mismatches.append((v, best, int(tr["best"]), "tie" if tie else f"{(qt - qd) / qd:.1e}"))
+            untied += 0 if tie else 1
+        # float64: the winner is optimal among the valid candidates
+        cands = _candidate_axes64(P, tr)
+        with np.errstate(invalid="ignore", divide="ignore"):
+            q = [(_quality64(P, c) if c is not None else np.inf) for c in cands]
+        qmin = min(x if np.isfinite(x) else np.inf for x in q)
+        kd = slab7(P.astype(np.float64))
+        al = kd[:, :3].max(0) - kd[:, :3].min(0)
+        qal = al[0] * al[1] + al[0] * al[2] + al[1] * al[2]
+        if best >= 0:
+            qbest = _quality64(P, ca[v, :9].reshape(3, 3).astype(np.float64))
+            rel = (qbest - min(qmin, qal)) / max(min(qmin, qal), 1e-30)
+        else:
+            rel = (qal - qmin) / max(qal, 1e-30)
+        excess.append(rel)
+    excess = np.array(excess)
+    worst = float(excess.max())
+    if kind == "ico":
+        assert worst <= 1e-5, worst
+    else:
+        # thin tube walls: a normal of a sliver triangle carries a large
+        # float32 rounding error, so float32 and float64 rank near-equal
+        # candidates differently on a few percent of the nodes
+        assert np.median(excess) <= 1e-5 and np.mean(excess <= 1e-3) >= 0.99 and worst <= 0.1
+    assert compared > I // 2
+    # every choice that is not identical is a tie, bar 0.1 % (icosphere) or
+    # 1 % (ill-conditioned tubes) of the nodes
+    others = [m for m in mismatches if m[3] not in ("tie", "threshold")]
+    assert untied <= (0.001 if kind == "ico" else 0.01) * compared, others[:20]
+    assert threshold <= 0.01 * compared, threshold
+    if kind == "ico":
+        assert same >= 0.999 * compared, (same, compared, mismatches[:20])
+    print(f"{kind}: {same}/{compared} identical choices, {len(mismatches) - untied - threshold} ties, "
+          f"{threshold} fallbacks at the 1e-6 thresholds, {untied} other "
+          f"({slab_ties} nodes with slab ties between distinct points skipped); "
+          f"worst float64 excess {worst:.2e}; mismatches {mismatches[:8]}")
+
+
+# ------------------------------------------------------ non-finite vertices --
+# The tree is built on the finite mesh, then one vertex is made non-finite.
+# K1 (transforms.cu:216-411) compares strictly from the first vertex on, so
+# over a node's vertices in DFS order the first slab value is kept if it is
+# NaN and NaN values are never taken otherwise; K3 (:855-897) reduces each
+# triangle from vertex 0 with kr_min / kr_max and lets a triangle's result
+# into the node's range only when it is strictly beyond it.
+
+def _kmin(a, b):
+    return np.where(a < b, a, b)
+
+
+def _kmax(a, b):
+    return np.where(a < b, b, a)
+
+
+@pytest.fixture(scope="module", params=["nan", "inf"])
+def bad_tree(request, oracle_lib):
+    V, F = synth.icosphere(6, seed=1)
+    nodes, parents, *_ = oracle_lib.lbvh_build(V, F)
+    clean = oracle_lib.obb_tree(nodes, parents, V, F, with_ids=True)
+    W = V.copy()
+    w = int(F[len(F) // 3, 1])  # the second vertex of a face in the middle of the mesh
+    W[w] = np.nan if request.param == "nan" else np.array([np.inf, 0.25, -np.inf], np.float32)
+    out = oracle_lib.obb_tree(nodes, parents, W, F, with_ids=True, with_ranges=True, with_axes=True)
+    return dict(V=W, F=F, nodes=nodes, w=w, clean=clean, out=out, kind=request.param)
+
+
+def test_nonfinite_vertex_kdop_ids(bad_tree):
+    V, F, nodes = bad_tree["V"], bad_tree["F"], bad_tree["nodes"]
+    ids = bad_tree["out"][2]
+    I = len(F) - 1
+    for v in range(2 * len(F) - 1):
+        vid = F[nodes["a"][subtree_leaves(nodes, v, I)]][:, :3].ravel()
+        with np.errstate(invalid="ignore"):
+            pr = slab7(V[vid])
+        for s in range(7):
+            col = pr[:, s]
+            if np.isnan(col[0]):
+                emn = emx = vid[0]
+            elif np.isnan(col).all():
+                continue
+            else:
+                emn, emx = vid[np.nanargmin(col)], vid[np.nanargmax(col)]
+            assert ids[v, s] == emn and ids[v, 7 + s] == emx, (v, s)
+
+
+def test_nonfinite_vertex_refit_ranges(bad_tree):
+    V, F, nodes, w = bad_tree["V"], bad_tree["F"], bad_tree["nodes"], bad_tree["w"]
+    obbs, mats, ids, ranges, ca = bad_tree["out"]
+    I = len(F) - 1
+    checked = 0
+    for v in range(I):
+        if ca[v, 9] != 0:
+            continue
+        ax = ca[v, :9].reshape(3, 3)
+        tri = F[nodes["a"][subtree_leaves(nodes, v, I)]][:, :3]
+        if not np.any(tri == w):
+            continue
+        with np.errstate(invalid="ignore", over="ignore"):
+            d = [(V[tri[:, j], None, 0] * ax[None, :, 0] + V[tri[:, j], None, 1] * ax[None, :, 1])
+                 + V[tri[:, j], None, 2] * ax[None, :, 2] for j in range(3)]
+            tmin = _kmin(_kmin(d[0], d[1]), d[2])
+            tmax = _kmax(_kmax(d[0], d[1]), d[2])
+            P = V[ids[v, _INTERLEAVE]]
+            dp = (P[:, None, 0] * ax[None, :, 0] + P[:, None, 1] * ax[None, :, 1]) + P[:, None, 2] * ax[None, :, 2]
+        # the 14 points start the range (their ranges are the candidate's,
+        # transforms.cu:749-755); NaN triangle results never enter
+        emin = np.minimum(dp.min(0), np.nanmin(np.where(np.isnan(tmin), np.inf, tmin), axis=0))
+        emax = np.maximum(dp.max(0), np.nanmax(np.where(np.isnan(tmax), -np.inf, tmax), axis=0))
+        assert np.array_equal(ranges[v, :3], emin) and np.array_equal(ranges[v, 3:], emax), v
+        checked += 1
+    # a NaN vertex is never one of a node's 14 points unless it comes first:
+    # nodes above it keep real candidates and exercise the triangle rule.  An
+    # infinite vertex is an extreme of every node above it, its base edge is
+    # not finite and every such node falls back to its AABB.
+    assert checked > 0 if bad_tree["kind"] == "nan" else checked == 0
+
+
+def test_nonfinite_vertex_is_local(bad_tree):
+    # nodes whose subtree does not hold the vertex keep their clean boxes
+    V, F, nodes, w = bad_tree["V"], bad_tree["F"], bad_tree["nodes"], bad_tree["w"]
+    cobbs, cmats, cids = bad_tree["clean"]
+    obbs, mats, ids = bad_tree["out"][:3]
+    I = len(F) - 1
+    touched = 0
+    for v in range(2 * len(F) - 1):
+        tri = F[nodes["a"][subtree_leaves(nodes, v, I)]][:, :3]
+        if np.any(tri == w):
+            touched += 1
+            continue
+        assert np.array_equal(obbs[v], cobbs[v]) and np.array_equal(mats[v], cmats[v]), v
+        assert np.array_equal(ids[v], cids[v]), v
+    assert 0 < touched < len(F)
