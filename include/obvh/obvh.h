@@ -178,6 +178,18 @@ kr_error obvh_dito_soa(const float* x, const float* y, const float* z, kr_size c
                        kr_obb3* out_obb, obvh_dito_trace* trace,
                        void* workspace, size_t workspace_bytes, void* stream);
 
+/* obvh_dito_soa without a workspace argument and without any allocation.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops).  Its
+ * workspace is a static 1 MiB device array inside libobvh.so, one per GPU,
+ * held until the process ends; count must not exceed
+ * OBVH_DITO_NOALLOC_MAX_POINTS (larger counts return an error).  Calls
+ * that overlap on one GPU share that array and race: serialise them (one
+ * stream, or events).  Results equal obvh_dito_soa's.                    */
+#define OBVH_DITO_NOALLOC_MAX_POINTS ((kr_size)1 << 28)
+kr_error obvh_dito_soa_noalloc(const float* x, const float* y, const float* z, kr_size count,
+                               kr_obb3* out_obb, obvh_dito_trace* trace, void* stream);
+
 /* Batched DiTO over `nsets` sets stored back to back in SoA arrays;
  * set s is points [offsets[s], offsets[s+1]).  offsets: device, nsets+1
  * entries.  tile_points: points per work tile (multiple of 512; 0 = auto).

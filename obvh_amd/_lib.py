@@ -17,6 +17,7 @@ EXPORTS = [
     "kr_cuda_bvh_obb_tree",
     "obvh_dito_workspace_size",
     "obvh_dito_soa",
+    "obvh_dito_soa_noalloc",
     "obvh_dito_batch_workspace_size",
     "obvh_dito_batch",
     "obvh_upload_sets",
@@ -82,6 +83,8 @@ def lib():
     L.obvh_dito_workspace_size.restype = sz
     L.obvh_dito_soa.argtypes = [vp, vp, vp, u64, vp, vp, vp, sz, vp]
     L.obvh_dito_soa.restype = C.c_char_p
+    L.obvh_dito_soa_noalloc.argtypes = [vp, vp, vp, u64, vp, vp, vp]
+    L.obvh_dito_soa_noalloc.restype = C.c_char_p
     L.obvh_dito_batch_workspace_size.argtypes = [u32, u64, u32]
     L.obvh_dito_batch_workspace_size.restype = sz
     L.obvh_dito_batch.argtypes = [vp, vp, vp, vp, u32, u64, vp, vp, u32, u32, vp, sz, vp]

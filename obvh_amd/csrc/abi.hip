@@ -67,6 +67,34 @@ kr_error fail(const char* msg) {
     return msg;
 }
 
+// Workspace of obvh_dito_soa_noalloc.  Each device holds its own copy at its
+// own address until the process ends; in host code `scratch` names a host
+// stand-in, so kernels get the device address from hipGetSymbolAddress.
+// 1 MiB holds the single-set DiTO workspace of up to
+// OBVH_DITO_NOALLOC_MAX_POINTS points (at most 4098 tiles of 140 B).
+constexpr size_t kNoAllocScratchBytes = (size_t)1 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_ptr;  // device address of `scratch`, per device
+
+// Address of the current device's `scratch`, looked up once per device.
+kr_error noalloc_scratch(void** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return obvh_hip_error(e, "obvh_dito_soa_noalloc: hipGetDevice");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return obvh_hip_error(e, "obvh_dito_soa_noalloc: hipGetSymbolAddress");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return nullptr;
+}
+
 }  // namespace
 
 void obvh_set_error(const char* msg) { g_last = msg; }
@@ -103,6 +131,25 @@ kr_error obvh_dito_soa(const float* x, const float* y, const float* z, kr_size c
     if (count > 0xFFFFFFF0ull) return fail("obvh_dito_soa: more than 2^32 - 16 points");
     TRY(obvh::dito_launch(x, y, z, nullptr, nullptr, count, 1, count, out_obb, trace, 0, 0, workspace,
                           workspace_bytes, (hipStream_t)stream));
+    return nullptr;
+}
+
+// The allocation-free single-set fit: obvh_dito_soa's kernels with the
+// workspace in the file-scope device array `scratch` (its address per device
+// from hipGetSymbolAddress, taken once per device on the first call).
+kr_error obvh_dito_soa_noalloc(const float* x, const float* y, const float* z, kr_size count, kr_obb3* out_obb,
+                               obvh_dito_trace* trace, void* stream) {
+    if (!out_obb) return fail("obvh_dito_soa_noalloc: out_obb is NULL");
+    if (count > 0 && (!x || !y || !z)) return fail("obvh_dito_soa_noalloc: NULL point array");
+    if (count > OBVH_DITO_NOALLOC_MAX_POINTS)
+        return fail("obvh_dito_soa_noalloc: more than OBVH_DITO_NOALLOC_MAX_POINTS points");
+    obvh::DitoWs wsd;
+    const size_t need = obvh::dito_carve(&wsd, nullptr, 0, 1, count, obvh::dito_auto_tile(1, count));
+    if (need > kNoAllocScratchBytes) return fail("obvh_dito_soa_noalloc: the static scratch array is too small");
+    void* ws = nullptr;
+    TRY(noalloc_scratch(&ws));
+    TRY(obvh::dito_launch(x, y, z, nullptr, nullptr, count, 1, count, out_obb, trace, 0, 0, ws,
+                          kNoAllocScratchBytes, (hipStream_t)stream));
     return nullptr;
 }
 

@@ -67,3 +67,17 @@ def test_product_path_fails_loudly_without_gpu():
 
     with pytest.raises(_lib.ObvhError):
         ops.dito(torch.zeros(4), torch.zeros(4), torch.zeros(4))
+
+
+def test_noalloc_dito_scratch_covers_its_limit():
+    """obvh_dito_soa_noalloc's static 1 MiB scratch holds obvh_dito_soa's
+    workspace for every count up to OBVH_DITO_NOALLOC_MAX_POINTS; larger
+    counts are rejected before any device call."""
+    L = _lib.lib()
+    limit = 1 << 28
+    ns = np.unique(np.concatenate([np.arange(0, 20000, 7), np.logspace(4, np.log10(limit), 4000).astype(np.int64),
+                                   [limit - 1, limit]]))
+    assert max(L.obvh_dito_workspace_size(int(n)) for n in ns) <= 1 << 20
+    one = C.c_void_p(16)
+    err = L.obvh_dito_soa_noalloc(one, one, one, limit + 1, one, None, None)
+    assert err and b"OBVH_DITO_NOALLOC_MAX_POINTS" in err
