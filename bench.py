@@ -491,6 +491,9 @@ def main() -> int:
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-2m", action="store_true", help="skip the 2M-triangle transform legs")
     ap.add_argument("--no-trace", action="store_true", help="skip the OBVH packing + traversal leg")
+    ap.add_argument("--only", choices=["c2", "c3"], default=None,
+                    help="time only one transform workload and print its leg as JSON (profiling runs): "
+                         "c2 = the icosphere LBVH, c3 = the 2M-triangle hairball ATRBVH")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run two 1 GiB device copies (calibration of the HBM byte counters)")
     ap.add_argument("--serve-seconds", type=float, default=60.0, help="open-loop serving duration (s)")
@@ -509,6 +512,11 @@ def main() -> int:
 
     torch.cuda.set_device(0)
     dev = torch.cuda.get_device_name(0)
+    if args.only:
+        V, F = synth.icosphere(args.freq, seed=2) if args.only == "c2" else synth.hairball(seed=3)
+        leg = transform_leg(V, F, args.only, args.steps, args.warmup, False, "lbvh" if args.only == "c2" else "atrbvh")
+        print(json.dumps({"only": args.only, **leg}))
+        return 0
     t_gen = time.perf_counter()
     V, F = synth.icosphere(args.freq, seed=2)
     print(f"[bench] device={dev}", file=sys.stderr, flush=True)

@@ -618,173 +618,227 @@ __device__ __forceinline__ void proj14(const f3* P, f3 d, bool sane, float& mn, 
     mx = b;
 }
 
-__device__ __forceinline__ float2 proj14v(const f3* P, f3 d, bool sane) {
-    float mn, mx;
-    proj14(P, d, sane, mn, mx);
-    return make_float2(mn, mx);
+// The candidate search over a point list P of NP points (NP a compile-time
+// count).  Reference mode (kRef, NP = 14): the node's 14 extremal points in
+// the reference's interleaved order, each projection range with the
+// compare-and-select fallback above.  Distinct mode: the node's distinct
+// extremal points, first occurrences in the interleaved order, padded to NP
+// with copies of the first point.  Every search over the list is a
+// first-index strict compare (a later copy of a point never wins) and every
+// range a min / max (the same value for a multiset and its set), so both
+// modes choose the same candidate with the same axes and lengths.  They can
+// differ only in the sign of a zero range end, which no decision uses
+// (lengths, qualities and their compares treat -0 as +0) and which the refit
+// replaces: the refit's range is the key minimum / maximum over every
+// subtree vertex, the extremal points included.  Distinct mode needs finite,
+// bounded points (v_min / v_max); other nodes take reference mode.
+template <int NP, bool kRef>
+struct Pts {
+    const f3* P;
+    bool sane;  // reference mode: the points are finite and bounded
+
+    __device__ __forceinline__ void proj(f3 d, float& mn, float& mx) const {
+        if (kRef) {
+            proj14(P, d, sane, mn, mx);
+            return;
+        }
+        float a = dot3(P[0], d), b = a;
+#pragma unroll
+        for (int i = 1; i < NP; ++i) {
+            const float p = dot3(P[i], d);
+            a = fminf(a, p);
+            b = fmaxf(b, p);
+        }
+        mn = a;
+        mx = b;
+    }
+    __device__ __forceinline__ float2 projv(f3 d) const {
+        float mn, mx;
+        proj(d, mn, mx);
+        return make_float2(mn, mx);
+    }
+};
+
+__device__ __forceinline__ f3 pick(bool c, f3 a, f3 b) { return f3{c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z}; }
+__device__ __forceinline__ float2 pick2(bool c, float2 a, float2 b) {
+    return make_float2(c ? a.x : b.x, c ? a.y : b.y);
 }
 
-// transforms.cu:484-544 with the normal's projection range (nmn, nmx) and
-// the edges' ranges pe[k] computed by the caller: each edge is shared by up
-// to three calls, so its range is computed once (identical arithmetic).
-__device__ __forceinline__ void from_normal_edges(const f3* P, f3 n, float nmn, float nmx, const f3 (&e)[3],
-                                                  const float2 (&pe)[3], bool sane, Best& b) {
+// One face of the ditetrahedron, transforms.cu:484-544: the normal's
+// projection range, then for each edge e[k] (its range pe[k] computed by
+// the caller: an edge is shared by up to three faces, its range is computed
+// once with identical arithmetic) the third axis m = e x n; a candidate
+// replaces the best one when its quality is strictly lower.
+template <int NP, bool kRef>
+__device__ __forceinline__ void face(const Pts<NP, kRef>& P, f3 n, float nmn, float nmx, f3 e0, f3 e1, f3 e2,
+                                     float2 pe0, float2 pe1, float2 pe2, Best& b) {
     const float leny = nmx - nmn;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const f3 m = vm::cross(e[k], n);
+        const f3 e = k == 0 ? e0 : (k == 1 ? e1 : e2);
+        const float2 pe = k == 0 ? pe0 : (k == 1 ? pe1 : pe2);
+        const f3 m = vm::cross(e, n);
         float mmn, mmx;
-        proj14(P, m, sane, mmn, mmx);
-        const float lenx = pe[k].y - pe[k].x;
+        P.proj(m, mmn, mmx);
+        const float lenx = pe.y - pe.x;
         const float lenz = mmx - mmn;
         const float q = quality(f3{lenx, leny, lenz});
         if (q < b.q) {
             b.q = q;
-            b.dmax = f3{pe[k].y, nmx, mmx};
-            b.dmin = f3{pe[k].x, nmn, mmn};
-            b.b0 = e[k];
+            b.dmax = f3{pe.y, nmx, mmx};
+            b.dmin = f3{pe.x, nmn, mmn};
+            b.b0 = e;
             b.b1 = n;
             b.b2 = m;
         }
     }
 }
 
-__device__ __forceinline__ void face(const f3* P, f3 n, const f3 (&e)[3], const float2 (&pe)[3], bool sane,
-                                     Best& b) {
-    float nmn, nmx;
-    proj14(P, n, sane, nmn, nmx);
-    from_normal_edges(P, n, nmn, nmx, e, pe, sane, b);
-}
-
-// The three tetrahedron faces on apex q of the base triangle (p0, p1, p2)
-// with edges e0, e1, e2 and their ranges pe (transforms.cu:641-700).
-__device__ __forceinline__ void apex_faces(const f3* P, f3 q, f3 p0, f3 p1, f3 p2, f3 e0, f3 e1, f3 e2,
-                                           float2 pe0, float2 pe1, float2 pe2, bool sane, Best& b) {
-    const f3 f0 = vm::normalize(sub3(q, p0));
-    const f3 f1 = vm::normalize(sub3(q, p1));
-    const f3 f2 = vm::normalize(sub3(q, p2));
-    const float2 pf0 = proj14v(P, f0, sane), pf1 = proj14v(P, f1, sane), pf2 = proj14v(P, f2, sane);
-    {
-        const f3 e[3] = {e0, f1, f0};
-        const float2 pe[3] = {pe0, pf1, pf0};
-        face(P, vm::normalize(vm::cross(f1, e0)), e, pe, sane, b);
-    }
-    {
-        const f3 e[3] = {e1, f2, f1};
-        const float2 pe[3] = {pe1, pf2, pf1};
-        face(P, vm::normalize(vm::cross(f2, e1)), e, pe, sane, b);
-    }
-    {
-        const f3 e[3] = {e2, f0, f2};
-        const float2 pe[3] = {pe2, pf0, pf2};
-        face(P, vm::normalize(vm::cross(f0, e2)), e, pe, sane, b);
-    }
-}
-
-__device__ __forceinline__ f3 pick(bool c, f3 a, f3 b) { return f3{c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z}; }
-
-__global__ __launch_bounds__(256, 4) void candidates(const kr_vec3* __restrict__ V, TreeWs ws, uint32_t I) {
-    const uint32_t node = blockIdx.x * blockDim.x + threadIdx.x;
-    if (node >= I) return;
-    if (*ws.error != 0 || !complete(ws.visit[node])) {
-        // malformed tree: never read the ids of a node that did not complete
-        if (!complete(ws.visit[node])) atomicOr(ws.error, 8u);
-        Cand c = {};
-        c.degen = 1;
-        ws.cand[node] = c;
-        return;
-    }
-    const float eps = 0.000001f;
-    Kdop k;
-    kdop_load(ws.kdop, node, k);
-    f3 P[14];
-#pragma unroll
-    for (int s = 0; s < 7; ++s) {
-        P[2 * s] = ldv(V, k.imn[s]);
-        P[2 * s + 1] = ldv(V, k.imx[s]);
-    }
-    // finite and below 2^61 in magnitude: no projection below can be NaN
-    uint32_t ex = 0;
-#pragma unroll
-    for (int i = 0; i < 14; ++i) {
-        ex = max(ex, __float_as_uint(P[i].x) & 0x7F800000u);
-        ex = max(ex, __float_as_uint(P[i].y) & 0x7F800000u);
-        ex = max(ex, __float_as_uint(P[i].z) & 0x7F800000u);
-    }
-    const bool sane = ex <= ((127u + 60u) << 23);
-    const f3 alen = f3{k.mx[0] - k.mn[0], k.mx[1] - k.mn[1], k.mx[2] - k.mn[2]};
-    Best b;
-    b.q = quality(alen);
-    b.b0 = f3{1, 0, 0};
-    b.b1 = f3{0, 1, 0};
-    b.b2 = f3{0, 0, 1};
-    b.dmin = f3{k.mn[0], k.mn[1], k.mn[2]};
-    b.dmax = f3{k.mx[0], k.mx[1], k.mx[2]};
-    int degen = 0;
-
-    // farthest slab pair (transforms.cu:437-456), first pair wins ties
-    f3 p0 = P[0], p1 = P[1];
+// Farthest slab pair (transforms.cu:437-456) over the interleaved list,
+// first pair wins ties: its index s.
+__device__ __forceinline__ int farthest_pair(const f3 (&P)[14]) {
+    int at = 0;
     float d2b = vm::dist2(P[0], P[1]);
 #pragma unroll
     for (int s = 1; s < 7; ++s) {
         const float d2 = vm::dist2(P[2 * s], P[2 * s + 1]);
         const bool c = d2 > d2b;
         d2b = c ? d2 : d2b;
-        p0 = pick(c, P[2 * s], p0);
-        p1 = pick(c, P[2 * s + 1], p1);
+        at = c ? s : at;
     }
+    return at;
+}
+
+// Base triangle on the farthest pair (p0, p1), then the ditetrahedron
+// (transforms.cu:546-674), over the list P.  The apex faces run as loops
+// (two apexes, three faces each, in the reference's candidate order), so
+// that the code per point count stays small.
+template <int NP, bool kRef>
+__device__ __forceinline__ void cand_search(const Pts<NP, kRef>& L, f3 p0, f3 p1, Best& b, int& degen) {
+    const float eps = 0.000001f;
+    const f3* P = L.P;
+    degen = 0;
     if (vm::dist2(p0, p1) < eps) {
         degen = 1;
+        return;
+    }
+    const f3 e0 = vm::normalize(sub3(p0, p1));
+    float dm = vm::dist_inf_edge(P[0], p0, e0);
+    f3 p2 = P[0];
+#pragma unroll
+    for (int i = 1; i < NP; ++i) {
+        const float d = vm::dist_inf_edge(P[i], p0, e0);
+        const bool c = d > dm;
+        dm = c ? d : dm;
+        p2 = pick(c, P[i], p2);
+    }
+    if (dm < eps) {
+        degen = 1;
+        return;
+    }
+    const f3 e1 = vm::normalize(sub3(p1, p2));
+    const f3 e2 = vm::normalize(sub3(p2, p0));
+    const f3 nn = vm::normalize(vm::cross(e1, e0));
+    const float2 pe0 = L.projv(e0), pe1 = L.projv(e1), pe2 = L.projv(e2);
+    // the normal's projections: its range, then the apexes (extreme points
+    // along n, first index wins, transforms.cu:609-628)
+    float pn[NP];
+#pragma unroll
+    for (int i = 0; i < NP; ++i) pn[i] = dot3(P[i], nn);
+    float nmn, nmx;
+    if (kRef) {
+        minmax14(*reinterpret_cast<const float(*)[14]>(pn), L.sane, nmn, nmx);
     } else {
-        const f3 e0 = vm::normalize(sub3(p0, p1));
-        float dm = vm::dist_inf_edge(P[0], p0, e0);
-        f3 p2 = P[0];
+        nmn = pn[0];
+        nmx = pn[0];
 #pragma unroll
-        for (int i = 1; i < 14; ++i) {
-            const float d = vm::dist_inf_edge(P[i], p0, e0);
-            const bool c = d > dm;
-            dm = c ? d : dm;
-            p2 = pick(c, P[i], p2);
-        }
-        if (dm < eps) {
-            degen = 1;
-        } else {
-            const f3 e1 = vm::normalize(sub3(p1, p2));
-            const f3 e2 = vm::normalize(sub3(p2, p0));
-            const f3 n = vm::normalize(vm::cross(e1, e0));
-            const float2 pe0 = proj14v(P, e0, sane), pe1 = proj14v(P, e1, sane), pe2 = proj14v(P, e2, sane);
-            float pn[14];
-#pragma unroll
-            for (int i = 0; i < 14; ++i) pn[i] = dot3(P[i], n);
-            {
-                float nmn, nmx;
-                minmax14(pn, sane, nmn, nmx);
-                const f3 e[3] = {e0, e1, e2};
-                const float2 pe[3] = {pe0, pe1, pe2};
-                from_normal_edges(P, n, nmn, nmx, e, pe, sane, b);
-            }
-            // apexes: extreme points along n, first index wins (transforms.cu:609-628)
-            float lo = pn[0], hi = pn[0];
-            f3 q1 = P[0], q0 = P[0];
-#pragma unroll
-            for (int i = 1; i < 14; ++i) {
-                const bool cl = pn[i] < lo, ch = pn[i] > hi;
-                lo = cl ? pn[i] : lo;
-                q1 = pick(cl, P[i], q1);
-                hi = ch ? pn[i] : hi;
-                q0 = pick(ch, P[i], q0);
-            }
-            const float tri = dot3(p0, n);
-            const bool q0v = hi - eps > tri;
-            const bool q1v = lo + eps < tri;
-            if (!q0v && !q1v) {
-                degen = 1;
-            } else {
-                if (q0v) apex_faces(P, q0, p0, p1, p2, e0, e1, e2, pe0, pe1, pe2, sane, b);
-                if (q1v) apex_faces(P, q1, p0, p1, p2, e0, e1, e2, pe0, pe1, pe2, sane, b);
-            }
+        for (int i = 1; i < NP; ++i) {
+            nmn = fminf(nmn, pn[i]);
+            nmx = fmaxf(nmx, pn[i]);
         }
     }
+    face(L, nn, nmn, nmx, e0, e1, e2, pe0, pe1, pe2, b);
+    float lo = pn[0], hi = pn[0];
+    f3 q1 = P[0], q0 = P[0];
+#pragma unroll
+    for (int i = 1; i < NP; ++i) {
+        const bool cl = pn[i] < lo, ch = pn[i] > hi;
+        lo = cl ? pn[i] : lo;
+        q1 = pick(cl, P[i], q1);
+        hi = ch ? pn[i] : hi;
+        q0 = pick(ch, P[i], q0);
+    }
+    const float tri = dot3(p0, nn);
+    const bool q0v = hi - eps > tri;
+    const bool q1v = lo + eps < tri;
+    if (!q0v && !q1v) {
+        degen = 1;
+        return;
+    }
+    // apex a = 0: q0 (candidates 3..11), a = 1: q1 (12..20); on apex q the
+    // faces j = 0, 1, 2 have edges (e_j, f_{j+1}, f_j) with f_i = q - p_i
+    // normalised and normal cross(f_{j+1}, e_j) normalised (transforms.cu:641-700)
+#pragma unroll 1
+    for (int a = 0; a < 2; ++a) {
+        if (!(a == 0 ? q0v : q1v)) continue;
+        const f3 q = pick(a == 0, q0, q1);
+        const f3 f0 = vm::normalize(sub3(q, p0));
+        const f3 f1 = vm::normalize(sub3(q, p1));
+        const f3 f2 = vm::normalize(sub3(q, p2));
+        const float2 pf0 = L.projv(f0), pf1 = L.projv(f1), pf2 = L.projv(f2);
+#pragma unroll 1
+        for (int j = 0; j < 3; ++j) {
+            const f3 ej = pick(j == 0, e0, pick(j == 1, e1, e2));
+            const float2 pej = pick2(j == 0, pe0, pick2(j == 1, pe1, pe2));
+            const f3 fa = pick(j == 0, f1, pick(j == 1, f2, f0));
+            const float2 pfa = pick2(j == 0, pf1, pick2(j == 1, pf2, pf0));
+            const f3 fb = pick(j == 0, f0, pick(j == 1, f1, f2));
+            const float2 pfb = pick2(j == 0, pf0, pick2(j == 1, pf1, pf2));
+            const f3 nf = vm::normalize(vm::cross(fa, ej));
+            float fmn, fmx;
+            L.proj(nf, fmn, fmx);
+            face(L, nf, fmn, fmx, ej, fa, fb, pej, pfa, pfb, b);
+        }
+    }
+}
+
+// Per internal node the candidate search (transforms.cu:677-756).  The
+// search's loops run over the node's distinct extremal points: on average
+// 5.4 of the 14 on a mesh LBVH (two triangles sharing an edge have four).  A
+// wave runs the loops for the largest distinct count among its lanes, so the
+// workgroup first sorts its 512 nodes by that count (LDS counting sort), lane
+// t takes the t-th node in that order, and the wave runs the search compiled
+// for 4, 6, 8 or 14 points.
+constexpr int kCandBlock = 256;
+
+struct CandTile {
+    uint32_t cid[kCandBlock][15];   // distinct extremal vertex ids in list order (odd stride)
+    uint32_t pair[kCandBlock][2];   // vertex ids of the farthest slab pair
+    uint8_t cnt[kCandBlock];        // number of distinct ids; 0 = nothing to search; 15 = reference mode
+    uint16_t order[kCandBlock];
+    uint32_t hist[16];
+};
+
+__device__ __forceinline__ void kdop_ids(const Kdop& k, uint32_t (&id)[14]) {
+#pragma unroll
+    for (int s = 0; s < 7; ++s) {
+        id[2 * s] = k.imn[s];
+        id[2 * s + 1] = k.imx[s];
+    }
+}
+
+// Finite and below 2^61 in magnitude: no projection can be NaN or overflow.
+__device__ __forceinline__ bool sane_points(const f3* P, int n) {
+    uint32_t ex = 0;
+    for (int i = 0; i < n; ++i) {
+        ex = max(ex, __float_as_uint(P[i].x) & 0x7F800000u);
+        ex = max(ex, __float_as_uint(P[i].y) & 0x7F800000u);
+        ex = max(ex, __float_as_uint(P[i].z) & 0x7F800000u);
+    }
+    return ex <= ((127u + 60u) << 23);
+}
+
+__device__ __forceinline__ void store_cand(TreeWs& ws, uint32_t node, const Best& b, int degen) {
     Cand c;
     c.b[0] = b.b0.x; c.b[1] = b.b0.y; c.b[2] = b.b0.z;
     c.b[3] = b.b1.x; c.b[4] = b.b1.y; c.b[5] = b.b1.z;
@@ -797,11 +851,139 @@ __global__ __launch_bounds__(256, 4) void candidates(const kr_vec3* __restrict__
     rk[3] = f2ord(b.dmax.x); rk[4] = f2ord(b.dmax.y); rk[5] = f2ord(b.dmax.z);
 }
 
+__device__ __forceinline__ Best aabb_best(const Kdop& k) {
+    const f3 alen = f3{k.mx[0] - k.mn[0], k.mx[1] - k.mn[1], k.mx[2] - k.mn[2]};
+    return Best{quality(alen), f3{1, 0, 0}, f3{0, 1, 0}, f3{0, 0, 1}, f3{k.mn[0], k.mn[1], k.mn[2]},
+                f3{k.mx[0], k.mx[1], k.mx[2]}};
+}
+
+// Reference mode over the node's 14 points in list order, for the rare
+// nodes with non-finite or huge extremal points (list kRedoList).  Run by
+// the spare workgroups of list_wide, after candidates: its registers do not
+// add to the distinct-mode search's.
+constexpr int kRedoList = kMaxPhases - 1;  // the climb's deferral lists use 0 and 1
+
+__device__ __forceinline__ void cand_ref(const kr_vec3* __restrict__ V, TreeWs& ws, uint32_t node) {
+    Kdop k;
+    kdop_load(ws.kdop, node, k);
+    uint32_t id[14];
+    kdop_ids(k, id);
+    f3 Q[14];
+#pragma unroll
+    for (int i = 0; i < 14; ++i) Q[i] = ldv(V, id[i]);
+    const int at = farthest_pair(Q);
+    f3 p0 = Q[0], p1 = Q[1];
+#pragma unroll
+    for (int s = 1; s < 7; ++s)
+        if (at == s) { p0 = Q[2 * s]; p1 = Q[2 * s + 1]; }
+    Best b = aabb_best(k);
+    int degen = 0;
+    const Pts<14, true> L{Q, sane_points(Q, 14)};
+    cand_search(L, p0, p1, b, degen);
+    store_cand(ws, node, b, degen);
+}
+
+// Distinct-mode search of one node with the list padded to NP points.
+template <int NP>
+__device__ __forceinline__ void cand_distinct(const kr_vec3* __restrict__ V, TreeWs& ws, const CandTile& T,
+                                              uint32_t s, uint32_t node, int cnt) {
+    f3 P[NP];
+    P[0] = ldv(V, T.cid[s][0]);
+#pragma unroll
+    for (int i = 1; i < NP; ++i) P[i] = i < cnt ? ldv(V, T.cid[s][i]) : P[0];  // copies of the first point pad the list
+    const float4* kd = &ws.kdop[(size_t)node * 7];
+    const float4 s0 = kd[0], s1 = kd[1], s2 = kd[2];
+    const f3 alen = f3{s0.z - s0.x, s1.z - s1.x, s2.z - s2.x};
+    Best b = {quality(alen), f3{1, 0, 0}, f3{0, 1, 0}, f3{0, 0, 1}, f3{s0.x, s1.x, s2.x}, f3{s0.z, s1.z, s2.z}};
+    const f3 p0 = ldv(V, T.pair[s][0]), p1 = ldv(V, T.pair[s][1]);
+    int degen = 0;
+    const Pts<NP, false> L{P, true};
+    cand_search(L, p0, p1, b, degen);
+    store_cand(ws, node, b, degen);
+}
+
+__global__ __launch_bounds__(kCandBlock, 4) void candidates(const kr_vec3* __restrict__ V, TreeWs ws, uint32_t I) {
+    __shared__ CandTile T;
+    const uint32_t t = threadIdx.x;
+    const uint32_t base = blockIdx.x * kCandBlock;
+    if (t < 16) T.hist[t] = 0;
+    const bool bad = *ws.error != 0;
+    __syncthreads();
+    // 1. the lane's own node: distinct ids and the farthest slab pair
+    {
+        const uint32_t node = base + t;
+        uint32_t cnt = 0;
+        if (node < I) {
+            if (bad || !complete(ws.visit[node])) {
+                // malformed tree: never read the ids of a node that did not complete
+                if (!complete(ws.visit[node])) atomicOr(ws.error, 8u);
+                Cand c = {};
+                c.degen = 1;
+                ws.cand[node] = c;
+            } else {
+                Kdop k;
+                kdop_load(ws.kdop, node, k);
+                uint32_t id[14];
+                kdop_ids(k, id);
+                f3 P[14];
+#pragma unroll
+                for (int i = 0; i < 14; ++i) P[i] = ldv(V, id[i]);
+                if (!sane_points(P, 14)) {  // reference mode, after this kernel
+                    const uint32_t slot = atomicAdd(&ws.nlist[kRedoList], 1u);
+                    if (slot < I) ws.list[(size_t)kRedoList * (I + 1) + slot] = node;
+                } else {
+                    const int at = farthest_pair(P);
+                    uint32_t a = id[0], b = id[1];
+#pragma unroll
+                    for (int s = 1; s < 7; ++s)
+                        if (at == s) { a = id[2 * s]; b = id[2 * s + 1]; }
+                    T.pair[t][0] = a;
+                    T.pair[t][1] = b;
+#pragma unroll
+                    for (int i = 0; i < 14; ++i) {
+                        bool dup = false;
+#pragma unroll
+                        for (int j = 0; j < i; ++j) dup |= id[j] == id[i];
+                        if (!dup) T.cid[t][cnt++] = id[i];
+                    }
+                }
+            }
+        }
+        T.cnt[t] = (uint8_t)cnt;
+        // counting sort by distinct count (rank within a count from an LDS
+        // atomic: the order among equal counts does not matter)
+        const uint32_t r = atomicAdd(&T.hist[cnt], 1u);
+        __syncthreads();
+        if (t == 0) {
+            uint32_t acc = 0;
+            for (int c = 0; c < 16; ++c) {
+                const uint32_t h = T.hist[c];
+                T.hist[c] = acc;
+                acc += h;
+            }
+        }
+        __syncthreads();
+        T.order[T.hist[cnt] + r] = (uint16_t)t;
+        __syncthreads();
+    }
+    // 2. the t-th node in count order, searched over the wave's point count
+    const uint32_t s = T.order[t];
+    const int cnt = T.cnt[s];
+    const int n = wave_max_i32(cnt);  // every lane is active here
+    if (cnt == 0) return;
+    const uint32_t node = base + s;
+    if (n <= 4) cand_distinct<4>(V, ws, T, s, node, cnt);
+    else if (n <= 6) cand_distinct<6>(V, ws, T, s, node, cnt);
+    else if (n <= 8) cand_distinct<8>(V, ws, T, s, node, cnt);
+    else cand_distinct<14>(V, ws, T, s, node, cnt);
+}
+
 // Wide nodes for the refit's tile-wide pass.  Workgroup w lists internal
 // nodes [w * chunk, (w + 1) * chunk): it counts its wide nodes, takes their
 // slots with one global atomic, then writes them (slots from an LDS
 // counter).  A few hundred workgroups keep the same-address atomics few.
 constexpr uint32_t kWideBlocks = 256;
+constexpr uint32_t kRedoBlocks = 8;  // workgroups for the reference-mode candidate search
 __device__ __forceinline__ bool wide_node(uint2 sp, bool& gap_wide) {
     const bool gap = (sp.y & kGap) != 0;
     const uint32_t hi = span_hi(sp), first = (sp.x + kRTile - 1) / kRTile * kRTile;
@@ -809,10 +991,20 @@ __device__ __forceinline__ bool wide_node(uint2 sp, bool& gap_wide) {
     return !gap && first <= hi && hi - first >= kRTile - 1;
 }
 
-__global__ __launch_bounds__(1024) void list_wide(TreeWs ws, uint32_t I) {
+// Workgroups wblocks and above run the candidate search in reference mode
+// on the nodes the distinct-mode search handed back.
+__global__ __launch_bounds__(1024) void list_wide(const kr_vec3* __restrict__ V, TreeWs ws, uint32_t I,
+                                                  uint32_t wblocks) {
     __shared__ uint32_t cnt, gap, base, off;
     const uint32_t t = threadIdx.x;
-    const uint32_t chunk = (I + gridDim.x - 1) / gridDim.x;
+    if (blockIdx.x >= wblocks) {
+        const uint32_t nredo = min(ws.nlist[kRedoList], I);
+        const uint32_t* redo = ws.list + (size_t)kRedoList * (I + 1);
+        for (uint32_t r = (blockIdx.x - wblocks) * blockDim.x + t; r < nredo; r += (gridDim.x - wblocks) * blockDim.x)
+            cand_ref(V, ws, redo[r]);
+        return;
+    }
+    const uint32_t chunk = (I + wblocks - 1) / wblocks;
     const uint32_t b0 = blockIdx.x * chunk, b1 = min(b0 + chunk, I);
     if (t == 0) { cnt = 0; gap = 0; off = 0; }
     __syncthreads();
@@ -1429,8 +1621,9 @@ kr_error tree_launch(const kr_bvh_node* nodes, const uint32_t* parents, const kr
         }
     }
     if (I > 0) {
-        hipLaunchKernelGGL(list_wide, dim3(min(kWideBlocks, (I + 1023) / 1024)), dim3(1024), 0, stream, ws, I);
-        hipLaunchKernelGGL(candidates, blocks(I, 256), dim3(256), 0, stream, V, ws, I);
+        hipLaunchKernelGGL(candidates, blocks(I, kCandBlock), dim3(kCandBlock), 0, stream, V, ws, I);
+        const uint32_t wb = min(kWideBlocks, (I + 1023) / 1024);
+        hipLaunchKernelGGL(list_wide, dim3(wb + kRedoBlocks), dim3(1024), 0, stream, V, ws, I, wb);
         hipLaunchKernelGGL(refit, blocks(N, kRTile), dim3(kRTile), 0, stream, nodes, parents, V, F, ws, N);
         hipLaunchKernelGGL(finalize, blocks(I, kStage), dim3(kStage), 0, stream, ws, obbs, mats, I);
     }
