@@ -17,7 +17,9 @@ cpu_baseline = the CPU oracle (a C port of the same method) on the same tree,
            ("parity").
 cpu_reference_transform = the reference's own CPU transform (kr_bvh_obb_tree,
            a full DiTO of every subtree; a different method) restated in C,
-           single thread, on the subtrees of the first 262144 leaves.
+           single thread, on the whole tree.
+roofline_c3 = the same roofline for the 2M-triangle hairball ATRBVH transform.
+host = the host's core count, CPU model and the threads the CPU legs used.
 Further legs (reported in the same JSON line): collapse + SAH of the tree,
 OBVH packing and closest-hit traversal of 1M camera and 1M random rays
 through the collapsed OBB tree, 2M-triangle transforms (LBVH and ATRBVH
@@ -51,6 +53,40 @@ def pmc_traffic(profile: str):
     except OSError:
         return None
     return None
+
+
+def pmc_value(profile: str, key: str):
+    """A `key: value` line of a committed profile summary, or None."""
+    path = os.path.join(ROOT, "profiles", profile)
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith(key + ":"):
+                    return float(line.split(":", 1)[1].strip())
+    except OSError:
+        return None
+    return None
+
+
+def host_info() -> dict:
+    """Core count, CPU model and the threads the CPU legs use."""
+    import oracle
+
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = None
+    return {"nproc": os.cpu_count(), "usable_cores": affinity, "cpu_model": model,
+            "cpu_threads_used": oracle.host_threads()}
 
 
 def copy_probe() -> int:
@@ -428,7 +464,10 @@ def transform_leg(V, F, label: str, steps: int, warmup: int, with_cpu: bool, bui
            "collapse_ms": round(collapse_ms, 5), "sah": sah,
            "ms_per_step": round(ms_step, 5), "device_ms": round(dev_ms_mean, 5),
            "Mnodes_per_s": round(value, 2), "bytes_per_call": balg, "achieved_GBs": round(achieved, 1),
-           "roofline_frac": round(achieved / HBM_PEAK_GBS, 4), "cpu_baseline": None, "parity": None}
+           "roofline_frac": round(achieved / HBM_PEAK_GBS, 4), "cpu_baseline": None, "parity": None,
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "bytes_per_call": balg,
+                        "device_ms": round(dev_ms_mean, 5)}}
     if with_cpu:
         import oracle
 
@@ -455,15 +494,13 @@ def transform_leg(V, F, label: str, steps: int, warmup: int, with_cpu: bool, bui
         print(f"[bench] cpu oracle {label}: {cpu_s:.2f}s single thread, {cpu_mt:.2f}s on {thr} threads, "
               f"parity={out['parity']}", file=sys.stderr, flush=True)
         # the reference's own CPU transform (kr_bvh_obb_tree: a full DiTO of
-        # every subtree), a different method, timed on the subtrees of the
-        # first 262144 leaves
-        le = min(N, 262144)
+        # every subtree), a different method, on the whole tree
         tc = time.perf_counter()
-        _, _, _, fitted = oracle.cpu_obb_tree(nodes_h, parents_h, V, F, 0, le)
+        _, _, _, fitted = oracle.cpu_obb_tree(nodes_h, parents_h, V, F)
         ref_s = time.perf_counter() - tc
         out["cpu_reference_transform"] = {
             "value": round(fitted / ref_s / 1e6, 4), "unit": "Mnodes/s", "cores": 1, "kind": "port",
-            "sample": f"kr_bvh_obb_tree restated: the {fitted} internal nodes below the first {le} leaves "
+            "sample": f"whole workload: kr_bvh_obb_tree restated, all {fitted} internal nodes "
                       f"({ref_s:.2f} s, single thread)",
         }
         print(f"[bench] reference CPU transform {label}: {fitted} nodes in {ref_s:.2f}s", file=sys.stderr,
@@ -557,6 +594,24 @@ def main() -> int:
             print(f"[bench] serving leg failed: {e!r}", file=sys.stderr, flush=True)
             served = {"error": repr(e)}
 
+    roofline_c3 = None
+    c3 = (extra or {}).get("hairball_2M_atrbvh")
+    if c3 and "roofline" in c3:
+        roofline_c3 = dict(c3["roofline"])
+        roofline_c3["traffic"] = pmc_traffic("transform_hairball2m_pmc.txt")
+        roofline_c3["workload"] = (f"ATRBVH (treelet 7, 2 passes) -> OBB-tree transform, hairball of 50000 tubes: "
+                                   f"{c3['triangles']} triangles, {c3['internal_nodes']} internal nodes")
+        roofline_c3["kernel"] = "obb_tree call (7 kernels), device time on the launch stream"
+    valu = None
+    vi, vus = (pmc_value("transform_icosphere224_valu.txt", "candidates_valu_wave_instructions_per_call"),
+               pmc_value("transform_icosphere224_valu.txt", "candidates_avg_us"))
+    if vi and vus:
+        # 1024 SIMDs x 2.4 GHz / 4 cycles per wave64 VALU instruction
+        peak = 1024 * 2.4e9 / 4
+        valu = {"bound": "valu_issue", "kernel": "candidates", "achieved": round(vi / (vus * 1e-6) / 1e9, 1),
+                "peak": round(peak / 1e9, 1), "unit": "G wave-instructions/s",
+                "frac": round(vi / (vus * 1e-6) / peak, 4), "source": "profiles/transform_icosphere224_valu.txt"}
+
     line = {
         "metric": "OBB-tree transform Mnodes/s; batched DiTO Gpts/s; served p50/p99 ms",
         "value": round(value, 2),
@@ -589,8 +644,11 @@ def main() -> int:
             "bytes_per_call": balg,
             "device_ms": round(dev_ms_mean, 5),
         },
+        "roofline_c3": roofline_c3,
+        "valu_roofline_candidates": valu,
         "cpu_baseline": cpu,
         "cpu_reference_transform": main_leg.get("cpu_reference_transform"),
+        "host": host_info(),
         "parity": parity,
         "lbvh_build_ms": main_leg["build_ms"],
         "collapse_ms": main_leg["collapse_ms"],
