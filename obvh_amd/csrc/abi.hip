@@ -26,6 +26,10 @@ struct DeviceCache {
     size_t pts_bytes = 0;
     void* ws = nullptr;
     size_t ws_bytes = 0;
+    void* dws = nullptr;   // workspace of kr_cuda_points_obb (its completion counters stay zero between calls)
+    size_t dws_bytes = 0;
+    bool dws_clear = true;  // counters not yet zeroed on this allocation
+    hipEvent_t ev[2] = {nullptr, nullptr};  // timing of kr_cuda_points_obb
     void* out = nullptr;  // kr_obb3 + error word
     void* trav = nullptr;  // traversal workspace of kr_cuda_soa_obvh_intersect
     size_t trav_bytes = 0;
@@ -77,14 +81,19 @@ static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
 std::mutex g_scratch_mu;
 std::vector<void*> g_scratch_ptr;  // device address of `scratch`, per device
+std::vector<char> g_scratch_used;  // the first call on a device zeroes the fit's completion counters
 
-// Address of the current device's `scratch`, looked up once per device.
-kr_error noalloc_scratch(void** out) {
+// Address of the current device's `scratch`, looked up once per device;
+// *first is true for the first call on the device.
+kr_error noalloc_scratch(void** out, bool* first) {
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return obvh_hip_error(e, "obvh_dito_soa_noalloc: hipGetDevice");
     std::lock_guard<std::mutex> lock(g_scratch_mu);
-    if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+    if ((int)g_scratch_ptr.size() <= dev) {
+        g_scratch_ptr.resize(dev + 1, nullptr);
+        g_scratch_used.resize(dev + 1, 0);
+    }
     if (!g_scratch_ptr[dev]) {
         void* p = nullptr;
         e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
@@ -92,7 +101,16 @@ kr_error noalloc_scratch(void** out) {
         g_scratch_ptr[dev] = p;
     }
     *out = g_scratch_ptr[dev];
+    *first = !g_scratch_used[dev];
     return nullptr;
+}
+
+// The two-launch fit ran on the current device's `scratch`: its counters are zero from now on.
+void noalloc_scratch_used() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((int)g_scratch_used.size() > dev) g_scratch_used[dev] = 1;
 }
 
 }  // namespace
@@ -129,8 +147,13 @@ kr_error obvh_dito_soa(const float* x, const float* y, const float* z, kr_size c
     if (!out_obb) return fail("obvh_dito_soa: out_obb is NULL");
     if (count > 0 && (!x || !y || !z)) return fail("obvh_dito_soa: NULL point array");
     if (count > 0xFFFFFFF0ull) return fail("obvh_dito_soa: more than 2^32 - 16 points");
-    TRY(obvh::dito_launch(x, y, z, nullptr, nullptr, count, 1, count, out_obb, trace, 0, 0, workspace,
-                          workspace_bytes, (hipStream_t)stream));
+    if (count == 0) {  // the empty set's box, written by the set-up kernel
+        TRY(obvh::dito_launch(x, y, z, nullptr, nullptr, count, 1, count, out_obb, trace, 0, 0, workspace,
+                              workspace_bytes, (hipStream_t)stream));
+        return nullptr;
+    }
+    TRY(obvh::dito_one_launch(x, y, z, nullptr, count, out_obb, trace, workspace, workspace_bytes, true,
+                              (hipStream_t)stream));
     return nullptr;
 }
 
@@ -147,9 +170,18 @@ kr_error obvh_dito_soa_noalloc(const float* x, const float* y, const float* z, k
     const size_t need = obvh::dito_carve(&wsd, nullptr, 0, 1, count, obvh::dito_auto_tile(1, count));
     if (need > kNoAllocScratchBytes) return fail("obvh_dito_soa_noalloc: the static scratch array is too small");
     void* ws = nullptr;
-    TRY(noalloc_scratch(&ws));
-    TRY(obvh::dito_launch(x, y, z, nullptr, nullptr, count, 1, count, out_obb, trace, 0, 0, ws,
-                          kNoAllocScratchBytes, (hipStream_t)stream));
+    bool first = false;
+    TRY(noalloc_scratch(&ws, &first));
+    if (count == 0) {
+        TRY(obvh::dito_launch(x, y, z, nullptr, nullptr, count, 1, count, out_obb, trace, 0, 0, ws,
+                              kNoAllocScratchBytes, (hipStream_t)stream));
+        return nullptr;
+    }
+    // the completion counters keep a fixed place in the array; the launches
+    // leave them zero, so only the first call on a device clears them
+    TRY(obvh::dito_one_launch(x, y, z, nullptr, count, out_obb, trace, ws, kNoAllocScratchBytes, first,
+                              (hipStream_t)stream));
+    if (first) noalloc_scratch_used();
     return nullptr;
 }
 
@@ -234,41 +266,46 @@ kr_obb3 kr_cuda_points_obb(const kr_vec3* points, kr_size count, kr_scalar* elap
     const uint32_t tile = obvh::dito_auto_tile(1, count);
     const size_t wsb = obvh::dito_carve(&wsd, nullptr, 0, 1, count, tile);
     hipError_t e;
+    const bool realloc_ws = c->dws_bytes < wsb || !c->dws;
     if ((e = grow(&c->pts, &c->pts_bytes, count * sizeof(kr_vec3))) != hipSuccess ||
-        (e = grow(&c->ws, &c->ws_bytes, wsb)) != hipSuccess) {
+        (e = grow(&c->dws, &c->dws_bytes, wsb)) != hipSuccess) {
         obvh_hip_error(e, "kr_cuda_points_obb: hipMalloc");
         return nan_box;
     }
+    if (realloc_ws) c->dws_clear = true;
     if (!c->out && (e = hipMalloc(&c->out, 256)) != hipSuccess) {
         obvh_hip_error(e, "kr_cuda_points_obb: hipMalloc");
         return nan_box;
+    }
+    for (auto& ev : c->ev) {
+        if (!ev && (e = hipEventCreate(&ev)) != hipSuccess) {
+            obvh_hip_error(e, "kr_cuda_points_obb: hipEventCreate");
+            return nan_box;
+        }
     }
     if ((e = hipMemcpy(c->pts, points, count * sizeof(kr_vec3), hipMemcpyHostToDevice)) != hipSuccess) {
         obvh_hip_error(e, "kr_cuda_points_obb: hipMemcpy H2D");
         return nan_box;
     }
-    hipEvent_t ev0, ev1;
-    hipEventCreate(&ev0);
-    hipEventCreate(&ev1);
-    hipEventRecord(ev0, nullptr);
-    kr_error r = obvh::dito_launch(nullptr, nullptr, nullptr, (const float*)c->pts, nullptr, count, 1, count,
-                                   (kr_obb3*)c->out, nullptr, tile, 0, c->ws, c->ws_bytes, nullptr);
-    hipEventRecord(ev1, nullptr);
+    hipEventRecord(c->ev[0], nullptr);
+    // two launches; a fresh workspace first gets its completion counters zeroed
+    kr_error r = obvh::dito_one_launch(nullptr, nullptr, nullptr, (const float*)c->pts, count, (kr_obb3*)c->out,
+                                       nullptr, c->dws, c->dws_bytes, c->dws_clear, nullptr);
+    hipEventRecord(c->ev[1], nullptr);
     kr_obb3 o = nan_box;
     if (!r) {
+        c->dws_clear = false;
         if ((e = hipMemcpy(&o, c->out, sizeof(kr_obb3), hipMemcpyDeviceToHost)) != hipSuccess) {
             obvh_hip_error(e, "kr_cuda_points_obb: hipMemcpy D2H");
             o = nan_box;
         } else if (elapsed_time) {
             float ms = 0.0f;
-            hipEventElapsedTime(&ms, ev0, ev1);
+            hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
             *elapsed_time = ms;
         }
     } else {
         g_last = (const char*)r;
     }
-    hipEventDestroy(ev0);
-    hipEventDestroy(ev1);
     return o;
 }
 

@@ -152,90 +152,99 @@ __global__ __launch_bounds__(256) void dito_tile_table(uint32_t nsets, DitoWs ws
 
 // --------------------------------------------------------------- pass 1 --
 
+// Slab extremes of points [base, end) of the set starting at sbeg: 14
+// (value, index within the set) pairs written to v_out / i_out by threads
+// 0..13 of the workgroup.
+template <class P>
+__device__ __forceinline__ void tile_pass1(P pts, uint64_t sbeg, uint64_t base, uint64_t end,
+                                           float (&sh_v)[kWaves][14], uint32_t (&sh_i)[kWaves][14],
+                                           float* v_out, uint32_t* i_out) {
+    const int lane = lane_id(), wave = wave_id();
+    const float NaN = __builtin_nanf("");
+    const uint32_t nchunks = (uint32_t)((end - base + kChunk - 1) / kChunk);
+    // per lane and slab side: running extreme and the index where it was
+    // first reached (points of one lane are visited in increasing index)
+    float rmn[7], rmx[7];
+    uint32_t imn[7], imx[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        rmn[k] = NaN;  // NaN never compares: the first real value wins
+        rmx[k] = NaN;
+        imn[k] = imx[k] = kNoIdx;
+    }
+    for (uint32_t c = wave; c < nchunks; c += kWaves) {
+        const uint64_t cb = base + (uint64_t)c * kChunk + lane;
+        const bool full = base + (uint64_t)(c + 1) * kChunk <= end;
+        f3 p[kG];
+#pragma unroll
+        for (int j = 0; j < kG; ++j) {
+            const uint64_t idx = cb + (uint64_t)j * kWave;
+            p[j] = (full || idx < end) ? pts.load(idx) : f3{NaN, NaN, NaN};
+        }
+        // Per point and slab side a strict compare against the running
+        // extreme keeps the first point that reaches it, as the sequential
+        // scan does; !(v >= cur) also takes the first value, while cur is
+        // NaN.  NaN values (the short last chunk's missing points, or
+        // non-finite input) are skipped here; the candidate step applies the
+        // reference scan's rule for a NaN at point 0.
+        const uint32_t i0 = (uint32_t)(cb - sbeg);
+#pragma unroll
+        for (int j = 0; j < kG; ++j) {
+            float pr[7];
+            slab7(p[j], pr);
+            const uint32_t idx = i0 + (uint32_t)j * kWave;
+#pragma unroll
+            for (int k = 0; k < 7; ++k) {
+                if (!(pr[k] >= rmn[k]) && pr[k] == pr[k]) { rmn[k] = pr[k]; imn[k] = idx; }
+                if (!(pr[k] <= rmx[k]) && pr[k] == pr[k]) { rmx[k] = pr[k]; imx[k] = idx; }
+            }
+        }
+    }
+    // wave: (value, index) lexicographic, then the workgroup over waves
+#pragma unroll
+    for (int side = 0; side < 14; ++side) {
+        const int k = side < 7 ? side : side - 7;
+        float v = side < 7 ? rmn[k] : rmx[k];
+        uint32_t i = side < 7 ? imn[k] : imx[k];
+        if (i == kNoIdx) v = side < 7 ? FLT_MAX * 2.0f : -FLT_MAX * 2.0f;
+        if (side < 7) wave_pair<true>(v, i); else wave_pair<false>(v, i);
+        if (lane == 0) {
+            sh_v[wave][side] = v;
+            sh_i[wave][side] = i;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 14) {
+        const int side = threadIdx.x;
+        float v = sh_v[0][side];
+        uint32_t i = sh_i[0][side];
+        for (int w = 1; w < kWaves; ++w) {
+            const float ov = sh_v[w][side];
+            const uint32_t oi = sh_i[w][side];
+            if (oi == kNoIdx) continue;
+            const bool better = (i == kNoIdx) ||
+                                (side < 7 ? pair_better<true>(v, i, ov, oi) : pair_better<false>(v, i, ov, oi));
+            if (better) { v = ov; i = oi; }
+        }
+        v_out[side] = v;
+        i_out[side] = i;
+    }
+    __syncthreads();
+}
+
 template <class P>
 __global__ __launch_bounds__(kBlock) void dito_pass1(P pts, const uint64_t* offs, uint32_t nsets,
                                                      uint32_t tile, DitoWs ws) {
     __shared__ float sh_v[kWaves][14];
     __shared__ uint32_t sh_i[kWaves][14];
     const uint32_t T = ws.first_tile[nsets];
-    const int lane = lane_id(), wave = wave_id();
-    const float NaN = __builtin_nanf("");
-
     for (uint32_t t = blockIdx.x; t < T; t += gridDim.x) {
         const uint32_t s = ws.tile_set[t];
         const uint64_t sbeg = offs[s], send = offs[s + 1];
         const uint32_t ft = ws.first_tile[s];
         const uint64_t base = sbeg + (uint64_t)(t - ft) * tile;
         const uint64_t end = base + tile < send ? base + tile : send;
-        const uint32_t nchunks = (uint32_t)((end - base + kChunk - 1) / kChunk);
-
-        // per lane and slab side: running extreme and the index where it was
-        // first reached (points of one lane are visited in increasing index)
-        float rmn[7], rmx[7];
-        uint32_t imn[7], imx[7];
-#pragma unroll
-        for (int k = 0; k < 7; ++k) {
-            rmn[k] = NaN;  // NaN never compares: the first real value wins
-            rmx[k] = NaN;
-            imn[k] = imx[k] = kNoIdx;
-        }
-        for (uint32_t c = wave; c < nchunks; c += kWaves) {
-            const uint64_t cb = base + (uint64_t)c * kChunk + lane;
-            const bool full = base + (uint64_t)(c + 1) * kChunk <= end;
-            f3 p[kG];
-#pragma unroll
-            for (int j = 0; j < kG; ++j) {
-                const uint64_t idx = cb + (uint64_t)j * kWave;
-                p[j] = (full || idx < end) ? pts.load(idx) : f3{NaN, NaN, NaN};
-            }
-            // Per point and slab side a strict compare against the running
-            // extreme keeps the first point that reaches it, as the sequential
-            // scan does; !(v >= cur) also takes the first value, while cur is
-            // NaN.  NaN values (the short last chunk's missing points, or
-            // non-finite input) are skipped: a NaN is never selected.
-            const uint32_t i0 = (uint32_t)(cb - sbeg);
-#pragma unroll
-            for (int j = 0; j < kG; ++j) {
-                float pr[7];
-                slab7(p[j], pr);
-                const uint32_t idx = i0 + (uint32_t)j * kWave;
-#pragma unroll
-                for (int k = 0; k < 7; ++k) {
-                    if (!(pr[k] >= rmn[k]) && pr[k] == pr[k]) { rmn[k] = pr[k]; imn[k] = idx; }
-                    if (!(pr[k] <= rmx[k]) && pr[k] == pr[k]) { rmx[k] = pr[k]; imx[k] = idx; }
-                }
-            }
-        }
-        // wave: (value, index) lexicographic, then the workgroup over waves
-#pragma unroll
-        for (int side = 0; side < 14; ++side) {
-            const int k = side < 7 ? side : side - 7;
-            float v = side < 7 ? rmn[k] : rmx[k];
-            uint32_t i = side < 7 ? imn[k] : imx[k];
-            if (i == kNoIdx) v = side < 7 ? FLT_MAX * 2.0f : -FLT_MAX * 2.0f;
-            if (side < 7) wave_pair<true>(v, i); else wave_pair<false>(v, i);
-            if (lane == 0) {
-                sh_v[wave][side] = v;
-                sh_i[wave][side] = i;
-            }
-        }
-        __syncthreads();
-        if (threadIdx.x < 14) {
-            const int side = threadIdx.x;
-            float v = sh_v[0][side];
-            uint32_t i = sh_i[0][side];
-            for (int w = 1; w < kWaves; ++w) {
-                const float ov = sh_v[w][side];
-                const uint32_t oi = sh_i[w][side];
-                if (oi == kNoIdx) continue;
-                const bool better = (i == kNoIdx) ||
-                                    (side < 7 ? pair_better<true>(v, i, ov, oi) : pair_better<false>(v, i, ov, oi));
-                if (better) { v = ov; i = oi; }
-            }
-            ws.part1_v[(size_t)t * 14 + side] = v;
-            ws.part1_i[(size_t)t * 14 + side] = i;
-        }
-        __syncthreads();
+        tile_pass1(pts, sbeg, base, end, sh_v, sh_i, &ws.part1_v[(size_t)t * 14], &ws.part1_i[(size_t)t * 14]);
     }
 }
 
@@ -413,37 +422,44 @@ __device__ void dito_candidate_search(const float mn[7], const float mx[7], cons
     tr.best = best_id;
 }
 
-// One wave per set: reduce the tile partials, then the wave searches.
+// One wave: reduce the nt tile partials of a set (points [sbeg, send),
+// first tile ft), then search; state and trace written by lane 0.
 template <class P>
-__global__ __launch_bounds__(kBlock) void dito_cand(P pts, const uint64_t* offs, uint32_t nsets, uint64_t total,
-                                                    DitoWs ws, obvh_dito_trace* trace) {
-    const uint32_t s = blockIdx.x * kWaves + wave_id();
-    if (s >= nsets) return;
-    const uint64_t sbeg = offs[s], send = offs[s + 1];
-    if (send == sbeg || !set_valid(sbeg, send, total)) return;
-    const uint32_t ft = ws.first_tile[s], nt = ws.first_tile[s + 1] - ft;
+__device__ __forceinline__ void set_cand(P pts, uint64_t sbeg, uint64_t send, uint32_t ft, uint32_t nt,
+                                         const float* part1_v, const uint32_t* part1_i, SetState* state,
+                                         obvh_dito_trace* trace) {
     const int lane = lane_id();
+    // Lane l reduces side l % 16 (sides 0..13) over the tiles j = l / 16
+    // (mod 4); two shuffle steps join the four groups.  (value, index) pairs
+    // order lexicographically (no NaN values: pass 1 never takes one), so
+    // the order of the reduction does not matter.
     float v[14];
     uint32_t ix[14];
-#pragma unroll
-    for (int side = 0; side < 14; ++side) {
-        v[side] = side < 7 ? FLT_MAX * 2.0f : -FLT_MAX * 2.0f;
-        ix[side] = kNoIdx;
-    }
-    for (uint32_t j = lane; j < nt; j += kWave) {
-#pragma unroll
-        for (int side = 0; side < 14; ++side) {
-            const float ov = ws.part1_v[(size_t)(ft + j) * 14 + side];
-            const uint32_t oi = ws.part1_i[(size_t)(ft + j) * 14 + side];
-            const bool better = (ix[side] == kNoIdx) ||
-                                (side < 7 ? pair_better<true>(v[side], ix[side], ov, oi)
-                                          : pair_better<false>(v[side], ix[side], ov, oi));
-            if (better) { v[side] = ov; ix[side] = oi; }
+    {
+        const int side = lane & 15;
+        const bool mn = side < 7;
+        float a = mn ? FLT_MAX * 2.0f : -FLT_MAX * 2.0f;
+        uint32_t ai = kNoIdx;
+        if (side < 14) {
+            for (uint32_t j = (uint32_t)(lane >> 4); j < nt; j += 4) {
+                const float ov = part1_v[(size_t)(ft + j) * 14 + side];
+                const uint32_t oi = part1_i[(size_t)(ft + j) * 14 + side];
+                const bool better =
+                    (ai == kNoIdx) || (mn ? pair_better<true>(a, ai, ov, oi) : pair_better<false>(a, ai, ov, oi));
+                if (better) { a = ov; ai = oi; }
+            }
         }
-    }
 #pragma unroll
-    for (int side = 0; side < 14; ++side) {
-        if (side < 7) wave_pair<true>(v[side], ix[side]); else wave_pair<false>(v[side], ix[side]);
+        for (int m = 16; m < kWave; m <<= 1) {
+            const float ov = __shfl_xor(a, m, kWave);
+            const uint32_t oi = (uint32_t)__shfl_xor((int)ai, m, kWave);
+            if (mn ? pair_better<true>(a, ai, ov, oi) : pair_better<false>(a, ai, ov, oi)) { a = ov; ai = oi; }
+        }
+#pragma unroll
+        for (int k = 0; k < 14; ++k) {
+            v[k] = __shfl(a, k, kWave);
+            ix[k] = (uint32_t)__shfl((int)ai, k, kWave);
+        }
     }
     const uint64_t n = send - sbeg;
     // The reference scan starts from point 0 and compares strictly
@@ -491,92 +507,107 @@ __global__ __launch_bounds__(kBlock) void dito_cand(P pts, const uint64_t* offs,
     SetState st = {};
     dito_candidate_search(mn, mx, minv, maxv, sel, st, tr);
     if (lane == 0) {
-        ws.state[s] = st;
-        trace[s] = tr;
+        *state = st;
+        *trace = tr;
     }
 }
 
+// One wave per set: reduce the tile partials, then the wave searches.
+template <class P>
+__global__ __launch_bounds__(kBlock) void dito_cand(P pts, const uint64_t* offs, uint32_t nsets, uint64_t total,
+                                                    DitoWs ws, obvh_dito_trace* trace) {
+    const uint32_t s = blockIdx.x * kWaves + wave_id();
+    if (s >= nsets) return;
+    const uint64_t sbeg = offs[s], send = offs[s + 1];
+    if (send == sbeg || !set_valid(sbeg, send, total)) return;
+    const uint32_t ft = ws.first_tile[s], nt = ws.first_tile[s + 1] - ft;
+    set_cand(pts, sbeg, send, ft, nt, ws.part1_v, ws.part1_i, &ws.state[s], &trace[s]);
+}
+
 // --------------------------------------------------------------- pass 2 --
+
+// Ranges of the points [base, end) on the set's three axes: 6 floats (min
+// b0..b2, max b0..b2) written to out6 by threads 0..5.
+template <class P>
+__device__ __forceinline__ void tile_pass2(P pts, uint64_t base, uint64_t end, const SetState* st,
+                                           float (&sh)[kWaves][6], float* out6) {
+    const int lane = lane_id(), wave = wave_id();
+    const float INF = FLT_MAX * 2.0f, NaN = __builtin_nanf("");
+    float r[6] = {INF, INF, INF, -INF, -INF, -INF};
+    if (st->mode != 1) {
+        const f3 b0 = f3{st->ax[0], st->ax[1], st->ax[2]};
+        const f3 b1 = f3{st->ax[3], st->ax[4], st->ax[5]};
+        const f3 b2 = f3{st->ax[6], st->ax[7], st->ax[8]};
+        const uint32_t nchunks = (uint32_t)((end - base + kChunk - 1) / kChunk);
+        for (uint32_t c = wave; c < nchunks; c += kWaves) {
+            const uint64_t cb = base + (uint64_t)c * kChunk + lane;
+            const bool full = base + (uint64_t)(c + 1) * kChunk <= end;
+            f3 p[kG];
+#pragma unroll
+            for (int j = 0; j < kG; ++j) {
+                const uint64_t idx = cb + (uint64_t)j * kWave;
+                p[j] = (full || idx < end) ? pts.load(idx) : f3{NaN, NaN, NaN};
+            }
+#pragma unroll
+            for (int j = 0; j < kG; ++j) {
+                const float d0 = dot3(p[j], b0), d1 = dot3(p[j], b1), d2 = dot3(p[j], b2);
+                r[0] = fminf(r[0], d0); r[3] = fmaxf(r[3], d0);
+                r[1] = fminf(r[1], d1); r[4] = fmaxf(r[4], d1);
+                r[2] = fminf(r[2], d2); r[5] = fmaxf(r[5], d2);
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { r[k] = wave_min(r[k]); r[3 + k] = wave_max(r[3 + k]); }
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) sh[wave][k] = r[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int k = threadIdx.x;
+        float a = sh[0][k];
+        for (int w = 1; w < kWaves; ++w) a = k < 3 ? fminf(a, sh[w][k]) : fmaxf(a, sh[w][k]);
+        out6[k] = a;
+    }
+    __syncthreads();
+}
 
 template <class P>
 __global__ __launch_bounds__(kBlock) void dito_pass2(P pts, const uint64_t* offs, uint32_t nsets,
                                                      uint32_t tile, DitoWs ws) {
     __shared__ float sh[kWaves][6];
     const uint32_t T = ws.first_tile[nsets];
-    const int lane = lane_id(), wave = wave_id();
-    const float INF = FLT_MAX * 2.0f, NaN = __builtin_nanf("");
-
     for (uint32_t t = blockIdx.x; t < T; t += gridDim.x) {
         const uint32_t s = ws.tile_set[t];
         const uint64_t sbeg = offs[s], send = offs[s + 1];
         const uint32_t ft = ws.first_tile[s];
         const uint64_t base = sbeg + (uint64_t)(t - ft) * tile;
         const uint64_t end = base + tile < send ? base + tile : send;
-        const SetState* st = &ws.state[s];
-        float r[6] = {INF, INF, INF, -INF, -INF, -INF};
-        if (st->mode != 1) {
-            const f3 b0 = f3{st->ax[0], st->ax[1], st->ax[2]};
-            const f3 b1 = f3{st->ax[3], st->ax[4], st->ax[5]};
-            const f3 b2 = f3{st->ax[6], st->ax[7], st->ax[8]};
-            const uint32_t nchunks = (uint32_t)((end - base + kChunk - 1) / kChunk);
-            for (uint32_t c = wave; c < nchunks; c += kWaves) {
-                const uint64_t cb = base + (uint64_t)c * kChunk + lane;
-                const bool full = base + (uint64_t)(c + 1) * kChunk <= end;
-                f3 p[kG];
-#pragma unroll
-                for (int j = 0; j < kG; ++j) {
-                    const uint64_t idx = cb + (uint64_t)j * kWave;
-                    p[j] = (full || idx < end) ? pts.load(idx) : f3{NaN, NaN, NaN};
-                }
-#pragma unroll
-                for (int j = 0; j < kG; ++j) {
-                    const float d0 = dot3(p[j], b0), d1 = dot3(p[j], b1), d2 = dot3(p[j], b2);
-                    r[0] = fminf(r[0], d0); r[3] = fmaxf(r[3], d0);
-                    r[1] = fminf(r[1], d1); r[4] = fmaxf(r[4], d1);
-                    r[2] = fminf(r[2], d2); r[5] = fmaxf(r[5], d2);
-                }
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < 3; ++k) { r[k] = wave_min(r[k]); r[3 + k] = wave_max(r[3 + k]); }
-        if (lane == 0) {
-#pragma unroll
-            for (int k = 0; k < 6; ++k) sh[wave][k] = r[k];
-        }
-        __syncthreads();
-        if (threadIdx.x < 6) {
-            const int k = threadIdx.x;
-            float a = sh[0][k];
-            for (int w = 1; w < kWaves; ++w) a = k < 3 ? fminf(a, sh[w][k]) : fmaxf(a, sh[w][k]);
-            ws.part2[(size_t)t * 6 + k] = a;
-        }
-        __syncthreads();
+        tile_pass2(pts, base, end, &ws.state[s], sh, &ws.part2[(size_t)t * 6]);
     }
 }
 
 // ---------------------------------------------------------------- final --
 
+// One wave: reduce the nt tile ranges of a set (first tile ft) and write
+// its box (lane 0).
 template <class P>
-__global__ __launch_bounds__(kBlock) void dito_final(P pts, const uint64_t* offs, uint32_t nsets, uint64_t total,
-                                                     DitoWs ws, kr_obb3* out, obvh_dito_trace* trace) {
-    const uint32_t s = blockIdx.x * kWaves + wave_id();
-    if (s >= nsets) return;
-    const uint64_t sbeg = offs[s], send = offs[s + 1];
-    if (send == sbeg || !set_valid(sbeg, send, total)) return;
-    const uint32_t ft = ws.first_tile[s], nt = ws.first_tile[s + 1] - ft;
+__device__ __forceinline__ void set_final(P pts, uint64_t sbeg, uint32_t ft, uint32_t nt, const float* part2,
+                                          const SetState* state, kr_obb3* out, obvh_dito_trace* trace) {
     const float INF = FLT_MAX * 2.0f;
     float a[6] = {INF, INF, INF, -INF, -INF, -INF};
     for (uint32_t j = lane_id(); j < nt; j += kWave) {
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
-            const float o = ws.part2[(size_t)(ft + j) * 6 + k];
+            const float o = part2[(size_t)(ft + j) * 6 + k];
             a[k] = k < 3 ? fminf(a[k], o) : fmaxf(a[k], o);
         }
     }
 #pragma unroll
     for (int k = 0; k < 3; ++k) { a[k] = wave_min(a[k]); a[3 + k] = wave_max(a[3 + k]); }
     if (lane_id() != 0) return;
-    const SetState st = ws.state[s];
+    const SetState st = *state;
     const f3 b0 = f3{st.ax[0], st.ax[1], st.ax[2]};
     const f3 b1 = f3{st.ax[3], st.ax[4], st.ax[5]};
     const f3 b2 = f3{st.ax[6], st.ax[7], st.ax[8]};
@@ -620,8 +651,72 @@ __global__ __launch_bounds__(kBlock) void dito_final(P pts, const uint64_t* offs
     o.ext.x = kmax(o.ext.x, 0.001f);
     o.ext.y = kmax(o.ext.y, 0.001f);
     o.ext.z = kmax(o.ext.z, 0.001f);
-    out[s] = o;
-    trace[s].result_aabb = result_aabb;
+    *out = o;
+    trace->result_aabb = result_aabb;
+}
+
+template <class P>
+__global__ __launch_bounds__(kBlock) void dito_final(P pts, const uint64_t* offs, uint32_t nsets, uint64_t total,
+                                                     DitoWs ws, kr_obb3* out, obvh_dito_trace* trace) {
+    const uint32_t s = blockIdx.x * kWaves + wave_id();
+    if (s >= nsets) return;
+    const uint64_t sbeg = offs[s], send = offs[s + 1];
+    if (send == sbeg || !set_valid(sbeg, send, total)) return;
+    const uint32_t ft = ws.first_tile[s], nt = ws.first_tile[s + 1] - ft;
+    set_final(pts, sbeg, ft, nt, ws.part2, &ws.state[s], &out[s], &trace[s]);
+}
+
+// ------------------------------------------------- one set, two launches --
+//
+// A single set of n > 0 points in tiles of `tile` points, one workgroup per
+// tile (grid = tile count).  Launch A: slab extremes per tile; the last
+// workgroup to finish (completion counter done[0]) reduces them and runs the
+// candidate search.  Launch B: axis ranges per tile; the last workgroup
+// (done[1]) reduces them and writes the box.  A workgroup publishes its tile
+// result with one agent-scope release (the atomic on the counter); the last
+// one acquires before reading the others'.  The counters must be zero before
+// the first call on a workspace (dito_one_clear, or a clear at allocation);
+// each last workgroup resets its counter, so later calls need no clear.
+
+__global__ void dito_one_clear(DitoWs ws) {
+    if (threadIdx.x < 2) ws.done[threadIdx.x] = 0;
+}
+
+template <class P>
+__global__ __launch_bounds__(kBlock) void dito_one_a(P pts, uint64_t n, uint32_t tile, DitoWs ws,
+                                                     obvh_dito_trace* trace) {
+    __shared__ float sh_v[kWaves][14];
+    __shared__ uint32_t sh_i[kWaves][14];
+    __shared__ uint32_t last;
+    const uint32_t t = blockIdx.x, T = gridDim.x;
+    const uint64_t base = (uint64_t)t * tile, end = base + tile < n ? base + tile : n;
+    // threads 0..13 (wave 0) write the tile's pairs; thread 0's release
+    // orders them before its count
+    tile_pass1(pts, 0, base, end, sh_v, sh_i, &ws.part1_v[(size_t)t * 14], &ws.part1_i[(size_t)t * 14]);
+    if (threadIdx.x == 0)
+        last = __hip_atomic_fetch_add(&ws.done[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == T - 1;
+    __syncthreads();
+    if (!last || wave_id() != 0) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    set_cand(pts, 0, n, 0, T, ws.part1_v, ws.part1_i, ws.state, trace);
+    if (lane_id() == 0) __hip_atomic_store(&ws.done[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <class P>
+__global__ __launch_bounds__(kBlock) void dito_one_b(P pts, uint64_t n, uint32_t tile, DitoWs ws, kr_obb3* out,
+                                                     obvh_dito_trace* trace) {
+    __shared__ float sh[kWaves][6];
+    __shared__ uint32_t last;
+    const uint32_t t = blockIdx.x, T = gridDim.x;
+    const uint64_t base = (uint64_t)t * tile, end = base + tile < n ? base + tile : n;
+    tile_pass2(pts, base, end, ws.state, sh, &ws.part2[(size_t)t * 6]);  // threads 0..5 write
+    if (threadIdx.x == 0)
+        last = __hip_atomic_fetch_add(&ws.done[1], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == T - 1;
+    __syncthreads();
+    if (!last || wave_id() != 0) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    set_final(pts, 0, 0, T, ws.part2, ws.state, out, trace);
+    if (lane_id() == 0) __hip_atomic_store(&ws.done[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ----------------------------------------------------------------- host --
@@ -634,6 +729,7 @@ size_t dito_carve(DitoWs* ws, void* base, size_t cap, uint32_t nsets, uint64_t t
     Carver c{(char*)base, 0, cap};
     const size_t T = dito_max_tiles(nsets, total, tile);
     ws->offsets_single = c.take<uint64_t>(2);
+    ws->done = c.take<uint32_t>(2);
     ws->first_tile = c.take<uint32_t>(nsets + 1);
     ws->tile_set = c.take<uint32_t>(T);
     ws->part1_v = c.take<float>(T * 14);
@@ -652,6 +748,38 @@ uint32_t dito_auto_tile(uint32_t nsets, uint64_t total) {
     if (t < (uint64_t)kChunk) t = kChunk;
     if (t > 65536) t = 65536;
     return (uint32_t)t;
+}
+
+template <class P>
+static void launch_one(P p, uint64_t n, uint32_t tile, uint32_t T, DitoWs ws, kr_obb3* out, obvh_dito_trace* tr,
+                       bool clear, hipStream_t stream) {
+    if (clear) hipLaunchKernelGGL(dito_one_clear, dim3(1), dim3(64), 0, stream, ws);
+    hipLaunchKernelGGL(dito_one_a<P>, dim3(T), dim3(kBlock), 0, stream, p, n, tile, ws, tr);
+    hipLaunchKernelGGL(dito_one_b<P>, dim3(T), dim3(kBlock), 0, stream, p, n, tile, ws, out, tr);
+}
+
+kr_error dito_one_launch(const float* x, const float* y, const float* z, const float* xyz, uint64_t n,
+                         kr_obb3* out, obvh_dito_trace* trace, void* workspace, size_t ws_bytes, bool clear,
+                         hipStream_t stream) {
+    if (n == 0) return "obvh_dito: the two-launch fit needs at least one point";
+    // Tiles of 2048 points (one chunk per wave) up to 1024 tiles, larger
+    // beyond: each workgroup pays one agent-scope release and the last one
+    // reduces every tile, so a small set wants few tiles.  At least the
+    // batched path's tile, so the workspace size of obvh_dito_workspace_size
+    // always suffices.
+    uint64_t tl = (n + 1023) / 1024;
+    tl = (tl + kChunk - 1) / kChunk * kChunk;
+    const uint32_t tile = (uint32_t)max(max(tl, (uint64_t)4 * kChunk), (uint64_t)dito_auto_tile(1, n));
+    DitoWs ws;
+    const size_t need = dito_carve(&ws, workspace, ws_bytes, 1, n, tile);
+    if (workspace == nullptr || ws_bytes < need) return "obvh_dito: workspace too small";
+    const uint64_t T = (n + tile - 1) / tile;
+    if (T > dito_max_tiles(1, n, tile)) return "obvh_dito: tile count";
+    obvh_dito_trace* tr = trace ? trace : ws.trace_tmp;
+    if (xyz) launch_one(PtsAoS{xyz}, n, tile, (uint32_t)T, ws, out, tr, clear, stream);
+    else launch_one(PtsSoA{x, y, z}, n, tile, (uint32_t)T, ws, out, tr, clear, stream);
+    OBVH_HIP_TRY(hipGetLastError());
+    return nullptr;
 }
 
 template <class P>

@@ -19,6 +19,7 @@ struct SetState {
 // Workspace pieces (carved from one caller buffer, see dito_carve).
 struct DitoWs {
     uint64_t* offsets_single;  // [0, count] for the single-set entry
+    uint32_t* done;            // 2 completion counters of the two-launch single-set fit
     uint32_t* first_tile;      // nsets + 1 (exclusive scan of tiles per set)
     uint32_t* tile_set;        // tile -> set
     float* part1_v;            // 14 per tile
@@ -38,5 +39,13 @@ kr_error dito_launch(const float* x, const float* y, const float* z, const float
                      const uint64_t* offsets, uint64_t single_count, uint32_t nsets, uint64_t total,
                      kr_obb3* out, obvh_dito_trace* trace, uint32_t tile, uint32_t workers,
                      void* workspace, size_t ws_bytes, hipStream_t stream);
+
+// One set of n > 0 points in two launches (slab pass + candidate search,
+// projection pass + box), each combined by the last workgroup to finish.
+// `clear` first zeroes the workspace's completion counters (needed before
+// the first call on a workspace; the launches leave them zero).
+kr_error dito_one_launch(const float* x, const float* y, const float* z, const float* xyz, uint64_t n,
+                         kr_obb3* out, obvh_dito_trace* trace, void* workspace, size_t ws_bytes, bool clear,
+                         hipStream_t stream);
 
 }  // namespace obvh
