@@ -515,6 +515,99 @@ def transform_leg(V, F, label: str, steps: int, warmup: int, with_cpu: bool, bui
     return out
 
 
+def tree_batch_leg(args) -> dict:
+    """1000 LBVH trees of 10k..200k triangles (log-uniform; noisy icospheres of
+    frequency round(sqrt(n / 20)), one mesh per frequency) through one
+    obvh_obb_tree_batch call, against 1000 single-tree calls
+    (obvh_obb_tree_async, one after another on one stream, shared workspace).
+    Every box and matrix of the batch is compared with the single-tree calls'
+    ones, and one tree per frequency with the oracle."""
+    import numpy as np
+    import torch
+
+    from obvh_amd import ops, synth
+
+    rng = np.random.default_rng(11)
+    n = np.exp(rng.uniform(np.log(10_000), np.log(200_000), size=args.trees))
+    freqs = np.clip(np.round(np.sqrt(n / 20.0)).astype(int), 1, None)
+    meshes, trees_by_f = {}, {}
+    for f in sorted(set(freqs.tolist())):
+        V, F = synth.icosphere(int(f), seed=int(f))
+        Vd = torch.from_numpy(V).cuda()
+        Fd = torch.from_numpy(F.view(np.int32)).cuda()
+        t = ops.lbvh_build(Vd, Fd)
+        meshes[f] = (V, F)
+        trees_by_f[f] = (t["nodes"], t["parents"], Vd, Fd)
+    trees = [trees_by_f[int(f)] for f in freqs]
+    b = ops.pack_trees(trees)
+    total, K = b["total"], b["ntrees"]
+    ws = torch.empty(ops.lib().obvh_obb_tree_batch_workspace_size(K, total), dtype=torch.uint8, device="cuda")
+    obbs = torch.empty((2 * total - K, 15), dtype=torch.float32, device="cuda")
+    mats = torch.empty((2 * total - K, 16), dtype=torch.float32, device="cuda")
+    status = torch.zeros(1, dtype=torch.int32, device="cuda")
+    ops.obb_tree_batch(b, workspace=ws, out=(obbs, mats), status=status)
+    torch.cuda.synchronize()
+    reps = 3
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    w0 = time.perf_counter()
+    e0.record()
+    for _ in range(reps):
+        ops.obb_tree_batch(b, workspace=ws, out=(obbs, mats))
+    e1.record()
+    torch.cuda.synchronize()
+    wall_b = (time.perf_counter() - w0) / reps
+    dev_b = e0.elapsed_time(e1) / reps / 1e3
+    # plain: one single-tree call per tree, outputs into the same layout
+    off = b["prim_offsets_host"]
+    nmax = int(max(F.shape[0] for F in (m[1] for m in meshes.values())))
+    ws1 = torch.empty(ops.lib().obvh_obb_tree_workspace_size(nmax, nmax - 1), dtype=torch.uint8, device="cuda")
+    po = torch.empty_like(obbs)
+    pm = torch.empty_like(mats)
+
+    def plain():
+        for t, (nd, par, Vd, Fd) in enumerate(trees):
+            s, e = 2 * int(off[t]) - t, 2 * int(off[t + 1]) - t - 1
+            ops.obb_tree(nd, par, Vd, Fd, workspace=ws1, out=(po[s:e], pm[s:e]))
+
+    plain()
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    e0.record()
+    plain()
+    e1.record()
+    torch.cuda.synchronize()
+    wall_p = time.perf_counter() - w0
+    dev_p = e0.elapsed_time(e1) / 1e3
+    same = bool(torch.equal(po, obbs) and torch.equal(pm, mats))
+    # one tree per frequency against the oracle
+    import oracle
+
+    thr = oracle.host_threads()
+    first = {}
+    for t, f in enumerate(freqs.tolist()):
+        first.setdefault(int(f), t)
+    oracle_ok = True
+    go, gm = obbs.cpu().numpy(), mats.cpu().numpy()
+    for f, t in first.items():
+        V, F = meshes[f]
+        nd, par, _, _ = trees_by_f[f]
+        eo, em = oracle.obb_tree(ops.nodes_to_numpy(nd), par.cpu().numpy().view(np.uint32), V, F, threads=thr)
+        s, e = 2 * int(off[t]) - t, 2 * int(off[t + 1]) - t - 1
+        oracle_ok &= bool(np.array_equal(go[s:e], eo) and np.array_equal(gm[s:e], em))
+    out = {"trees": K, "triangles": total, "internal_nodes": total - K,
+           "batched": {"ms": round(dev_b * 1e3, 3), "wall_ms": round(wall_b * 1e3, 3),
+                       "trees_per_s": round(K / dev_b, 1), "Mnodes_per_s": round((total - K) / dev_b / 1e6, 1)},
+           "plain_per_tree": {"ms": round(dev_p * 1e3, 3), "wall_ms": round(wall_p * 1e3, 3),
+                              "trees_per_s": round(K / max(dev_p, wall_p), 1)},
+           "batched_over_plain": round(max(dev_p, wall_p) / max(dev_b, wall_b), 2),
+           "status": int(status.item()), "equal_to_plain": same, "oracle_parity": oracle_ok,
+           "distinct_meshes": len(first)}
+    print(f"[bench] tree batch: {K} trees, {total} triangles: batched {dev_b*1e3:.2f} ms, plain "
+          f"{max(dev_p, wall_p)*1e3:.2f} ms ({out['batched_over_plain']}x), equal={same}, oracle={oracle_ok}",
+          file=sys.stderr, flush=True)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -528,7 +621,9 @@ def main() -> int:
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-2m", action="store_true", help="skip the 2M-triangle transform legs")
     ap.add_argument("--no-trace", action="store_true", help="skip the OBVH packing + traversal leg")
-    ap.add_argument("--only", choices=["c2", "c3"], default=None,
+    ap.add_argument("--trees", type=int, default=1000, help="batched transform: number of trees")
+    ap.add_argument("--no-tree-batch", action="store_true", help="skip the batched tree transform leg")
+    ap.add_argument("--only", choices=["c2", "c3", "trees"], default=None,
                     help="time only one transform workload and print its leg as JSON (profiling runs): "
                          "c2 = the icosphere LBVH, c3 = the 2M-triangle hairball ATRBVH")
     ap.add_argument("--copy-probe", action="store_true",
@@ -549,6 +644,9 @@ def main() -> int:
 
     torch.cuda.set_device(0)
     dev = torch.cuda.get_device_name(0)
+    if args.only == "trees":
+        print(json.dumps({"only": "trees", **tree_batch_leg(args)}))
+        return 0
     if args.only:
         V, F = synth.icosphere(args.freq, seed=2) if args.only == "c2" else synth.hairball(seed=3)
         leg = transform_leg(V, F, args.only, args.steps, args.warmup, False, "lbvh" if args.only == "c2" else "atrbvh")
@@ -577,6 +675,14 @@ def main() -> int:
             except Exception as e:
                 print(f"[bench] {name} leg failed: {e!r}", file=sys.stderr, flush=True)
                 extra[name] = {"error": repr(e)}
+
+    tree_batch = None
+    if not args.no_tree_batch:
+        try:
+            tree_batch = tree_batch_leg(args)
+        except Exception as e:
+            print(f"[bench] tree batch leg failed: {e!r}", file=sys.stderr, flush=True)
+            tree_batch = {"error": repr(e)}
 
     dito = None
     if not args.no_dito:
@@ -655,6 +761,7 @@ def main() -> int:
         "sah": main_leg["sah"],
         "traversal": main_leg.get("traversal"),
         "transform_2m": extra,
+        "tree_batch": tree_batch,
         "dito": dito,
         "serving": served,
     }

@@ -245,6 +245,32 @@ kr_error obvh_obb_tree_status_async(const void* workspace, size_t workspace_byte
                                     u32 primitive_count, u32* status, void* stream);
 kr_error obvh_obb_tree_status_string(u32 status);
 
+/* Many trees in one transform (no reference counterpart; the batched form
+ * of kr_cuda_bvh_obb_tree, transforms.cu:966-1059, for serving loops).
+ * Tree t has N_t = primitive_offsets[t+1] - primitive_offsets[t] >= 1
+ * primitives; its 2 N_t - 1 nodes and parents start at index
+ * 2 primitive_offsets[t] - t of `nodes` / `parents` (the single-tree layout,
+ * indices local to the tree, root at its local node 0), its N_t faces at
+ * primitive_offsets[t] of `faces` with vertex ids local to its vertices,
+ * which start at vertex_offsets[t] of `vertices`.  Both offset arrays are
+ * device arrays of ntrees + 1 entries; total_primitives is the sum of N_t
+ * (host).  obbs / transforms receive every tree's boxes and matrices in
+ * the layout of `nodes`, each equal to what kr_cuda_bvh_obb_tree computes
+ * for that tree alone.  Stream ordered, no allocation, graph capturable.
+ * Errors in the trees (as kr_cuda_bvh_obb_tree reports them, plus an empty
+ * tree, indices outside their tree) reach the caller through
+ * obvh_obb_tree_batch_status_async; obvh_obb_tree_status_string names them. */
+size_t obvh_obb_tree_batch_workspace_size(u32 ntrees, u32 total_primitives);
+kr_error obvh_obb_tree_batch(
+    const kr_bvh_node* nodes, const u32* parents,
+    const kr_vec3* vertices, const kr_uvec4* faces,
+    const u32* primitive_offsets, const u32* vertex_offsets,
+    u32 ntrees, u32 total_primitives,
+    kr_obb3* obbs, kr_mat4* transforms,
+    void* workspace, size_t workspace_bytes, void* stream);
+kr_error obvh_obb_tree_batch_status_async(const void* workspace, size_t workspace_bytes,
+                                          u32 total_primitives, u32* status, void* stream);
+
 /* Host-side tree validation (parents/children consistent, single root at
  * node 0, every node reachable once).  Inputs are HOST arrays.  Returns
  * NULL if the tree is well formed.  Used before launching the bottom-up

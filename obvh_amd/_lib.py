@@ -26,6 +26,9 @@ EXPORTS = [
     "obvh_obb_tree_async",
     "obvh_obb_tree_status_async",
     "obvh_obb_tree_status_string",
+    "obvh_obb_tree_batch_workspace_size",
+    "obvh_obb_tree_batch",
+    "obvh_obb_tree_batch_status_async",
     "obvh_validate_tree_host",
     "obvh_lbvh_workspace_size",
     "obvh_lbvh_build",
@@ -101,6 +104,12 @@ def lib():
     L.obvh_obb_tree_status_async.restype = C.c_char_p
     L.obvh_obb_tree_status_string.argtypes = [u32]
     L.obvh_obb_tree_status_string.restype = C.c_char_p
+    L.obvh_obb_tree_batch_workspace_size.argtypes = [u32, u32]
+    L.obvh_obb_tree_batch_workspace_size.restype = sz
+    L.obvh_obb_tree_batch.argtypes = [vp, vp, vp, vp, vp, vp, u32, u32, vp, vp, vp, sz, vp]
+    L.obvh_obb_tree_batch.restype = C.c_char_p
+    L.obvh_obb_tree_batch_status_async.argtypes = [vp, sz, u32, vp, vp]
+    L.obvh_obb_tree_batch_status_async.restype = C.c_char_p
     L.obvh_validate_tree_host.argtypes = [vp, vp, u32, u32]
     L.obvh_validate_tree_host.restype = C.c_char_p
     L.obvh_lbvh_workspace_size.argtypes = [u32]

@@ -373,6 +373,33 @@ kr_error obvh_obb_tree_status_async(const void* workspace, size_t workspace_byte
 
 kr_error obvh_obb_tree_status_string(u32 status) { return obvh::tree_status_string(status); }
 
+size_t obvh_obb_tree_batch_workspace_size(u32 ntrees, u32 total_primitives) {
+    (void)ntrees;
+    if (total_primitives == 0) return 256;
+    return obvh::batch_workspace_size(total_primitives);
+}
+
+kr_error obvh_obb_tree_batch(const kr_bvh_node* nodes, const u32* parents, const kr_vec3* vertices,
+                             const kr_uvec4* faces, const u32* primitive_offsets, const u32* vertex_offsets,
+                             u32 ntrees, u32 total_primitives, kr_obb3* obbs, kr_mat4* transforms, void* workspace,
+                             size_t workspace_bytes, void* stream) {
+    if (ntrees == 0 || total_primitives == 0) return nullptr;
+    if (total_primitives > 0x7FFFFFFFu) return fail("obvh_obb_tree_batch: too many primitives");
+    if (!nodes || !parents || !vertices || !faces || !primitive_offsets || !vertex_offsets || !obbs || !transforms)
+        return fail("obvh_obb_tree_batch: NULL array");
+    TRY(obvh::tree_batch_launch(nodes, parents, vertices, faces, primitive_offsets, vertex_offsets, ntrees,
+                                total_primitives, obbs, transforms, workspace, workspace_bytes, (hipStream_t)stream));
+    return nullptr;
+}
+
+kr_error obvh_obb_tree_batch_status_async(const void* workspace, size_t workspace_bytes, u32 total_primitives,
+                                          u32* status, void* stream) {
+    if (!status) return fail("obvh_obb_tree_batch_status_async: status is NULL");
+    if (total_primitives == 0) return fail("obvh_obb_tree_batch_status_async: no batch has zero primitives");
+    TRY(obvh::tree_batch_status_launch(workspace, workspace_bytes, total_primitives, status, (hipStream_t)stream));
+    return nullptr;
+}
+
 kr_error obvh_validate_tree_host(const kr_bvh_node* nodes, const u32* parents, u32 primitive_count,
                                  u32 internal_count) {
     if (primitive_count == 0) return nullptr;

@@ -135,6 +135,59 @@ __device__ __forceinline__ void obb_row(float* d, f3 mid, f3 v0, f3 v1, f3 v2, f
     d[12] = ext.x; d[13] = ext.y; d[14] = ext.z;
 }
 
+// ------------------------------------------------------ batch map --
+//
+// Numbering of a batch of trees joined into one combined tree (see the batch
+// section below).
+
+struct BatchMap {
+    const uint32_t* prim_off;  // K + 1 leaf offsets L_t
+    uint32_t K;
+    uint32_t Nsum;             // sum of N_t
+};
+
+// Tree of input node g: the last t with 2 L_t - t <= g.
+__device__ __forceinline__ uint32_t batch_tree_of_node(const BatchMap& m, uint32_t g) {
+    uint32_t lo = 0, hi = m.K;  // nb(lo) <= g < nb(hi)
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (2u * m.prim_off[mid] - mid <= g) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+__device__ __forceinline__ uint32_t batch_tree_of_leaf(const BatchMap& m, uint32_t f) {
+    uint32_t lo = 0, hi = m.K;  // L(lo) <= f < L(hi)
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (m.prim_off[mid] <= f) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+// Combined index of local node j of tree t (N leaves, first leaf Lt); j must be < 2N - 1.
+__device__ __forceinline__ uint32_t batch_map(const BatchMap& m, uint32_t t, uint32_t Lt, uint32_t N, uint32_t j) {
+    if (j + 1 < N) return Lt + j + (t + 1 < m.K ? 1u : 0u);
+    return (m.Nsum - 1) + Lt + (j - (N - 1));
+}
+// Output index of combined node c (a box the caller receives), or kInvalid
+// for an unused slot.  K == 0: no batch, the identity.
+__device__ __forceinline__ uint32_t batch_out_index(const BatchMap& m, uint32_t c) {
+    if (m.K == 0) return c;
+    const uint32_t Ic = m.Nsum - 1;
+    if (c >= Ic) {  // leaf position c - Ic
+        const uint32_t pos = c - Ic;
+        const uint32_t t = batch_tree_of_leaf(m, pos);
+        const uint32_t Lt = m.prim_off[t], N = m.prim_off[t + 1] - Lt;
+        return 2 * Lt - t + (N - 1) + (pos - Lt);
+    }
+    const uint32_t t = batch_tree_of_leaf(m, c);
+    const uint32_t Lt = m.prim_off[t];
+    if (t + 1 < m.K) {
+        if (c == Lt) return kInvalid;  // unused slot
+        return 2 * Lt - t + (c - Lt - 1);
+    }
+    return 2 * Lt - t + (c - Lt);
+}
+
 // Output staging for 256 consecutive nodes: boxes at row stride 15 floats,
 // matrices at row stride 17 (odd strides: no LDS bank conflicts), written
 // back to HBM as contiguous dwords.
@@ -142,6 +195,7 @@ constexpr int kStage = 256;
 struct Stage {
     float ob[kStage * 15];
     float mt[kStage * 17];
+    uint32_t orow[kStage];  // output node of each row (batches)
 };
 
 __device__ __forceinline__ void stage_nan(Stage& st, uint32_t r) {
@@ -152,12 +206,30 @@ __device__ __forceinline__ void stage_nan(Stage& st, uint32_t r) {
     for (int q = 0; q < 16; ++q) st.mt[r * 17 + q] = nan;
 }
 
-__device__ __forceinline__ void stage_flush(const Stage& st, kr_obb3* obbs, kr_mat4* mats, size_t first,
-                                            uint32_t cnt) {
-    float* go = reinterpret_cast<float*>(obbs) + first * 15;
-    for (uint32_t j = threadIdx.x; j < cnt * 15; j += kStage) go[j] = st.ob[j];
-    float* gm = reinterpret_cast<float*>(mats) + first * 16;
-    for (uint32_t j = threadIdx.x; j < cnt * 16; j += kStage) gm[j] = st.mt[(j >> 4) * 17 + (j & 15)];
+// Rows first .. first + cnt - 1 (combined indices) to their output nodes: the
+// same node indices, or for a batch each tree's own layout (runs of
+// consecutive rows stay consecutive; top nodes are dropped).
+__device__ __forceinline__ void stage_flush(Stage& st, kr_obb3* obbs, kr_mat4* mats, size_t first, uint32_t cnt,
+                                            const BatchMap& m) {
+    if (m.K == 0) {
+        float* go = reinterpret_cast<float*>(obbs) + first * 15;
+        for (uint32_t j = threadIdx.x; j < cnt * 15; j += kStage) go[j] = st.ob[j];
+        float* gm = reinterpret_cast<float*>(mats) + first * 16;
+        for (uint32_t j = threadIdx.x; j < cnt * 16; j += kStage) gm[j] = st.mt[(j >> 4) * 17 + (j & 15)];
+        return;
+    }
+    if (threadIdx.x < cnt) st.orow[threadIdx.x] = batch_out_index(m, (uint32_t)first + threadIdx.x);
+    __syncthreads();
+    float* go = reinterpret_cast<float*>(obbs);
+    for (uint32_t j = threadIdx.x; j < cnt * 15; j += kStage) {
+        const uint32_t r = j / 15, o = st.orow[r];
+        if (o != kInvalid) go[(size_t)o * 15 + (j - r * 15)] = st.ob[j];
+    }
+    float* gm = reinterpret_cast<float*>(mats);
+    for (uint32_t j = threadIdx.x; j < cnt * 16; j += kStage) {
+        const uint32_t o = st.orow[j >> 4];
+        if (o != kInvalid) gm[(size_t)o * 16 + (j & 15)] = st.mt[(j >> 4) * 17 + (j & 15)];
+    }
 }
 
 // ------------------------------------------------------------ k-DOP --
@@ -209,6 +281,19 @@ __device__ __forceinline__ void leaf_obb(f3 v0, f3 v1, f3 v2, float* ob, float* 
     obb_matrix(a0, a1, a2, m, ext, M);
 #pragma unroll
     for (int q = 0; q < 16; ++q) mt[q] = M[q];
+}
+
+// Arrival word of an unused internal slot of a batch: complete, tagged so
+// that the per-node kernels skip it (arrival tags never sum to this).
+constexpr uint32_t kHoleTag = 0xFFFFFFFFu;
+__device__ __forceinline__ bool hole(uint64_t v) { return (uint32_t)(v >> 32) == kHoleTag; }
+
+__global__ void batch_holes(TreeWs ws, BatchMap m) {
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k + 1 < m.K; k += gridDim.x * blockDim.x) {
+        const uint32_t idx = m.prim_off[k];
+        ws.visit[idx] = ((uint64_t)kHoleTag << 32) | 2ull;
+        ws.span[idx] = make_uint2(1u, 0u);  // empty: never listed as a wide node
+    }
 }
 
 __global__ void tree_clear(uint64_t* visit, uint32_t n, TreeWs ws) {
@@ -288,7 +373,7 @@ __device__ __forceinline__ bool leaf_kdop_global(const kr_bvh_node* nodes, const
 __global__ __launch_bounds__(kStage) void leaf_boxes(const kr_bvh_node* __restrict__ nodes,
                                                      const kr_vec3* __restrict__ V,
                                                      const kr_uvec4* __restrict__ F, kr_obb3* obbs,
-                                                     kr_mat4* mats, uint32_t N) {
+                                                     kr_mat4* mats, uint32_t N, BatchMap bm) {
     __shared__ Stage st;
     const uint32_t I = N - 1;
     const uint32_t base = blockIdx.x * kStage, pos = base + threadIdx.x;
@@ -303,7 +388,7 @@ __global__ __launch_bounds__(kStage) void leaf_boxes(const kr_bvh_node* __restri
         }
     }
     __syncthreads();
-    stage_flush(st, obbs, mats, (size_t)I + base, min((uint32_t)kStage, N - base));
+    stage_flush(st, obbs, mats, (size_t)I + base, min((uint32_t)kStage, N - base), bm);
 }
 
 // Phase 0 of the bottom-up k-DOP pass.  Workgroup w owns leaf positions
@@ -374,7 +459,9 @@ __global__ __launch_bounds__(kTile) void climb_leaves(const kr_bvh_node* __restr
         const uint32_t l = nodes[pos].u.children.left, r = nodes[pos].u.children.right;
         L.kids[t] = make_uint2(l, r);
         L.par_int[t] = parents[pos];
-        if (l >= total || r >= total || parents[l] != pos || parents[r] != pos) atomicOr(ws.error, 2u);
+        // (an unused slot of a batch has no children: no check)
+        if ((l >= total || r >= total || parents[l] != pos || parents[r] != pos) && !hole(ws.visit[pos]))
+            atomicOr(ws.error, 2u);
     }
     L.arrive[t] = 0;
     __syncthreads();
@@ -914,9 +1001,14 @@ __global__ __launch_bounds__(kCandBlock, 4) void candidates(const kr_vec3* __res
         const uint32_t node = base + t;
         uint32_t cnt = 0;
         if (node < I) {
-            if (bad || !complete(ws.visit[node])) {
+            const uint64_t vw = ws.visit[node];
+            if (hole(vw)) {
+                Cand c = {};
+                c.degen = 1;
+                ws.cand[node] = c;
+            } else if (bad || !complete(vw)) {
                 // malformed tree: never read the ids of a node that did not complete
-                if (!complete(ws.visit[node])) atomicOr(ws.error, 8u);
+                if (!complete(vw)) atomicOr(ws.error, 8u);
                 Cand c = {};
                 c.degen = 1;
                 ws.cand[node] = c;
@@ -1461,14 +1553,16 @@ __global__ __launch_bounds__(kRTile) void refit(const kr_bvh_node* __restrict__ 
 // LDS (odd row strides, no bank conflicts) and written back as contiguous
 // dwords.
 
-__global__ __launch_bounds__(kStage) void finalize(TreeWs ws, kr_obb3* obbs, kr_mat4* mats, uint32_t I) {
+__global__ __launch_bounds__(kStage) void finalize(TreeWs ws, kr_obb3* obbs, kr_mat4* mats, uint32_t I,
+                                                   BatchMap bm) {
     __shared__ Stage st;
     const uint32_t base = blockIdx.x * kStage;
     const uint32_t node = base + threadIdx.x;
     float* ob = &st.ob[threadIdx.x * 15];
     float* mt = &st.mt[threadIdx.x * 17];
     if (node < I) {
-        if (*ws.error != 0 || !complete(ws.visit[node])) {
+        const uint64_t vw = ws.visit[node];
+        if (*ws.error != 0 || !complete(vw) || hole(vw)) {
             stage_nan(st, threadIdx.x);
         } else {
             const float4* kd = &ws.kdop[(size_t)node * 7];
@@ -1506,7 +1600,7 @@ __global__ __launch_bounds__(kStage) void finalize(TreeWs ws, kr_obb3* obbs, kr_
         }
     }
     __syncthreads();
-    stage_flush(st, obbs, mats, base, min((uint32_t)kStage, I - min(base, I)));
+    stage_flush(st, obbs, mats, base, min((uint32_t)kStage, I - min(base, I)), bm);
 }
 
 __global__ void copy_ids(const kr_bvh_node* __restrict__ nodes, const kr_vec3* __restrict__ V,
@@ -1583,7 +1677,8 @@ kr_error side_stream(SideStream** out) {
 
 kr_error tree_launch(const kr_bvh_node* nodes, const uint32_t* parents, const kr_vec3* V,
                      const kr_uvec4* F, kr_obb3* obbs, kr_mat4* mats, uint32_t* ids, uint32_t N,
-                     void* workspace, size_t ws_bytes, hipStream_t stream) {
+                     void* workspace, size_t ws_bytes, hipStream_t stream, const BatchMap* batch) {
+    const BatchMap bm = batch ? *batch : BatchMap{nullptr, 0, 0};
     if (N == 0) return nullptr;
     TreeWs ws;
     size_t need = tree_carve(&ws, workspace, ws_bytes, N);
@@ -1594,6 +1689,8 @@ kr_error tree_launch(const kr_bvh_node* nodes, const uint32_t* parents, const kr
     SideStream* side = nullptr;
     if (kr_error e = side_stream(&side)) return e;
     hipLaunchKernelGGL(tree_clear, dim3(1024), dim3(256), 0, stream, ws.visit, I + 1, ws);
+    if (bm.K > 1)
+        hipLaunchKernelGGL(batch_holes, dim3(min((bm.K + 255) / 256, 1024u)), dim3(256), 0, stream, ws, bm);
     // climb: 512-leaf tiles, then 16384-leaf workgroups, then one workgroup
     // for the rest
     hipLaunchKernelGGL(climb_leaves, blocks(N, kTile), dim3(kTile), 0, stream, nodes, parents, V, F, ws, N);
@@ -1605,7 +1702,7 @@ kr_error tree_launch(const kr_bvh_node* nodes, const uint32_t* parents, const kr
     auto fork_leaf_boxes = [&]() -> kr_error {
         OBVH_HIP_TRY(hipEventRecord(side->fork, stream));
         OBVH_HIP_TRY(hipStreamWaitEvent(side->s, side->fork, 0));
-        hipLaunchKernelGGL(leaf_boxes, blocks(N, kStage), dim3(kStage), 0, side->s, nodes, V, F, obbs, mats, N);
+        hipLaunchKernelGGL(leaf_boxes, blocks(N, kStage), dim3(kStage), 0, side->s, nodes, V, F, obbs, mats, N, bm);
         OBVH_HIP_TRY(hipEventRecord(side->join, side->s));
         return nullptr;
     };
@@ -1625,11 +1722,165 @@ kr_error tree_launch(const kr_bvh_node* nodes, const uint32_t* parents, const kr
         const uint32_t wb = min(kWideBlocks, (I + 1023) / 1024);
         hipLaunchKernelGGL(list_wide, dim3(wb + kRedoBlocks), dim3(1024), 0, stream, V, ws, I, wb);
         hipLaunchKernelGGL(refit, blocks(N, kRTile), dim3(kRTile), 0, stream, nodes, parents, V, F, ws, N);
-        hipLaunchKernelGGL(finalize, blocks(I, kStage), dim3(kStage), 0, stream, ws, obbs, mats, I);
+        hipLaunchKernelGGL(finalize, blocks(I, kStage), dim3(kStage), 0, stream, ws, obbs, mats, I, bm);
     }
     if (ids)
         hipLaunchKernelGGL(copy_ids, blocks(2 * N - 1, 256), dim3(256), 0, stream, nodes, V, F, ws, ids, 2 * N - 1);
     OBVH_HIP_TRY(hipStreamWaitEvent(stream, side->join, 0));
+    OBVH_HIP_TRY(hipGetLastError());
+    return nullptr;
+}
+
+// ------------------------------------------------------------ batch --
+//
+// Many trees in one transform.  The batch's trees (tree t: N_t >= 1 leaves,
+// nodes, parents, faces and vertices back to back, indices local to the
+// tree) become one forest in the single-tree numbering, so that every
+// kernel runs once for the whole batch.  Each node's k-DOP, candidate axes,
+// refit range and box depend only on its own subtree, so the forest gives
+// every tree's nodes exactly the per-tree results.  The numbering keeps each
+// tree Karras-aligned: with L_t the first leaf position of tree t (prefix
+// sum of N_s), its internal node j goes to L_t + j + 1 (t < K - 1) or
+// L_t + j (the last tree) and its leaf i to (sum N - 1) + L_t + i; the
+// internal slots L_k (k < K - 1) stay unused (holes: marked complete with
+// tag kHoleTag, skipped by the per-node kernels).  Every tree root has no
+// parent.  Tree t's nodes start at 2 L_t - t in the input layout.
+
+// One lane per input node: write the combined node and parent.  Bad local
+// indices (children, parents, face ids outside the tree, an empty tree) set
+// the batch error word and are mapped out of range, where the transform's
+// own checks report them.
+__global__ void batch_gather_nodes(BatchMap m, const kr_bvh_node* __restrict__ nodes,
+                                   const uint32_t* __restrict__ parents, kr_bvh_node* cn, uint32_t* cp,
+                                   uint32_t* berr, uint32_t total_in) {
+    const uint32_t I = m.Nsum - 1, total = 2 * m.Nsum - 1;
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total_in; g += gridDim.x * blockDim.x) {
+        const uint32_t t = batch_tree_of_node(m, g);
+        const uint32_t Lt = m.prim_off[t], N = m.prim_off[t + 1] - Lt;
+        if (N == 0) { atomicOr(berr, 1u); continue; }
+        const uint32_t nb = 2 * Lt - t, j = g - nb, nn = 2 * N - 1;
+        if (j >= nn) { atomicOr(berr, 1u); continue; }
+        const uint32_t c = batch_map(m, t, Lt, N, j);
+        kr_bvh_node o = nodes[g];
+        if (j + 1 < N) {
+            const uint32_t l = o.u.children.left, r = o.u.children.right;
+            if (l >= nn || r >= nn) atomicOr(berr, 2u);
+            o.u.children.left = l < nn ? batch_map(m, t, Lt, N, l) : total;
+            o.u.children.right = r < nn ? batch_map(m, t, Lt, N, r) : total;
+        } else {
+            const uint32_t f = o.u.primitivesOffset;
+            if (f >= N) atomicOr(berr, 4u);
+            o.u.primitivesOffset = f < N ? Lt + f : 0xFFFFFFFFu;
+        }
+        cn[c] = o;
+        const uint32_t p = parents[g];
+        uint32_t cpar;
+        if (p == kInvalid) cpar = j == 0 ? kInvalid : total;  // only the root has none
+        else cpar = p < nn ? batch_map(m, t, Lt, N, p) : total;
+        if ((p == kInvalid) != (j == 0) || (p != kInvalid && p >= nn)) atomicOr(berr, 2u);
+        cp[c] = cpar;
+    }
+    // the unused internal slots L_k (k < K - 1) between the trees
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k + 1 < m.K; k += gridDim.x * blockDim.x) {
+        const uint32_t idx = m.prim_off[k];
+        cn[idx] = kr_bvh_node{};
+        cp[idx] = kInvalid;
+    }
+    (void)I;
+}
+
+// One lane per face: vertex ids rebased to the batch's vertex array.
+__global__ void batch_gather_faces(BatchMap m, const uint32_t* __restrict__ vert_off,
+                                   const kr_uvec4* __restrict__ faces, kr_uvec4* cf, uint32_t* berr) {
+    for (uint32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < m.Nsum; f += gridDim.x * blockDim.x) {
+        const uint32_t t = batch_tree_of_leaf(m, f);
+        const uint32_t vb = vert_off[t], nv = vert_off[t + 1] - vb;
+        kr_uvec4 a = faces[f];
+        if (a.x >= nv || a.y >= nv || a.z >= nv) {
+            atomicOr(berr, 8u);  // rebased to vertex 0 of the batch: never read outside the array
+            a.x = a.x < nv ? vb + a.x : 0u;
+            a.y = a.y < nv ? vb + a.y : 0u;
+            a.z = a.z < nv ? vb + a.z : 0u;
+        } else {
+            a.x += vb;
+            a.y += vb;
+            a.z += vb;
+        }
+        cf[f] = a;
+    }
+}
+
+struct BatchWs {
+    kr_bvh_node* nodes;
+    uint32_t* parents;
+    kr_uvec4* faces;
+    uint32_t* berr;
+    void* tree_ws;
+    size_t tree_ws_bytes;
+};
+
+size_t batch_carve(BatchWs* w, void* base, size_t cap, uint32_t Nsum) {
+    Carver c{(char*)base, 0, cap};
+    const size_t total = 2 * (size_t)Nsum - 1;
+    w->berr = c.take<uint32_t>(4);
+    w->nodes = c.take<kr_bvh_node>(total);
+    w->parents = c.take<uint32_t>(total);
+    w->faces = c.take<kr_uvec4>(Nsum);
+    TreeWs tw;
+    w->tree_ws_bytes = tree_carve(&tw, nullptr, 0, Nsum);
+    w->tree_ws = c.take<char>(w->tree_ws_bytes);
+    return c.used;
+}
+
+size_t batch_workspace_size(uint32_t Nsum) {
+    BatchWs w;
+    return batch_carve(&w, nullptr, 0, Nsum);
+}
+
+__global__ void batch_clear(uint32_t* berr) {
+    if (threadIdx.x == 0) *berr = 0;
+}
+
+kr_error tree_batch_launch(const kr_bvh_node* nodes, const uint32_t* parents, const kr_vec3* V, const kr_uvec4* F,
+                           const uint32_t* prim_off, const uint32_t* vert_off, uint32_t K, uint32_t Nsum,
+                           kr_obb3* obbs, kr_mat4* mats, void* workspace, size_t ws_bytes, hipStream_t stream) {
+    if (K == 0 || Nsum == 0) return nullptr;
+    if (Nsum < K) return "obvh_obb_tree_batch: every tree needs at least one primitive";
+    BatchWs w;
+    const size_t need = batch_carve(&w, workspace, ws_bytes, Nsum);
+    if (workspace == nullptr || ws_bytes < need) return "obvh_obb_tree_batch: workspace too small";
+    const BatchMap m{prim_off, K, Nsum};
+    const uint32_t total_in = 2 * Nsum - K;
+    auto grid = [](uint32_t n) { return dim3(min((n + 255) / 256, 65536u)); };
+    hipLaunchKernelGGL(batch_clear, dim3(1), dim3(64), 0, stream, w.berr);
+    hipLaunchKernelGGL(batch_gather_nodes, grid(max(total_in, K)), dim3(256), 0, stream, m, nodes, parents, w.nodes,
+                       w.parents, w.berr, total_in);
+    hipLaunchKernelGGL(batch_gather_faces, grid(Nsum), dim3(256), 0, stream, m, vert_off, F, w.faces, w.berr);
+    OBVH_HIP_TRY(hipGetLastError());
+    // leaf_boxes and finalize write every node's box and matrix straight to
+    // its place in the caller's per-tree layout
+    return tree_launch(w.nodes, w.parents, V, w.faces, obbs, mats, nullptr, Nsum, w.tree_ws, w.tree_ws_bytes, stream,
+                       &m);
+}
+
+__global__ void copy_batch_status(const uint32_t* berr, const uint32_t* error, uint32_t* status) {
+    const uint32_t b = *berr;
+    uint32_t s = *error;
+    if (b & 1u) s |= 16u;  // an empty tree or a node outside its tree
+    if (b & 2u) s |= 2u;
+    if (b & 4u) s |= 4u;
+    if (b & 8u) s |= 32u;
+    *status = s;
+}
+
+kr_error tree_batch_status_launch(const void* workspace, size_t ws_bytes, uint32_t Nsum, uint32_t* status,
+                                  hipStream_t stream) {
+    BatchWs w;
+    const size_t need = batch_carve(&w, const_cast<void*>(workspace), ws_bytes, Nsum);
+    if (workspace == nullptr || ws_bytes < need) return "obvh_obb_tree_batch_status: workspace too small";
+    TreeWs tw;
+    tree_carve(&tw, w.tree_ws, w.tree_ws_bytes, Nsum);
+    hipLaunchKernelGGL(copy_batch_status, dim3(1), dim3(1), 0, stream, w.berr, tw.error, status);
     OBVH_HIP_TRY(hipGetLastError());
     return nullptr;
 }
@@ -1649,6 +1900,8 @@ const char* tree_status_string(uint32_t err) {
     if (err & 1u) return "obvh_obb_tree: parent index out of range or cycle";
     if (err & 4u) return "obvh_obb_tree: leaf face id out of range";
     if (err & 8u) return "obvh_obb_tree: internal node never completed (malformed tree)";
+    if (err & 16u) return "obvh_obb_tree_batch: empty tree in the batch or node outside its tree";
+    if (err & 32u) return "obvh_obb_tree_batch: face vertex id outside its tree's vertices";
     if (err != 0u) return "obvh_obb_tree: unknown device error";
     return nullptr;
 }

@@ -41,14 +41,25 @@ constexpr uint32_t kRTile = 512;  // leaves per refit workgroup (tiles aligned t
 static_assert(2 + kMaxPhases <= 6, "error words 6 and 7 hold the wide-node count and gap flag");
 
 size_t tree_carve(TreeWs* ws, void* base, size_t cap, uint32_t N);
+struct BatchMap;
 kr_error tree_launch(const kr_bvh_node* nodes, const uint32_t* parents, const kr_vec3* V,
                      const kr_uvec4* F, kr_obb3* obbs, kr_mat4* mats, uint32_t* ids, uint32_t N,
-                     void* workspace, size_t ws_bytes, hipStream_t stream);
+                     void* workspace, size_t ws_bytes, hipStream_t stream, const BatchMap* batch = nullptr);
 
 // Copy the call's device error word (0 = success) into `status` (device or
 // pinned host memory) on `stream`, and its meaning.
 kr_error tree_status_launch(const void* workspace, size_t ws_bytes, uint32_t N, uint32_t* status,
                             hipStream_t stream);
 const char* tree_status_string(uint32_t err);
+
+// Many trees in one transform (transform.hip, "batch").
+struct BatchWs;
+size_t batch_carve(BatchWs* w, void* base, size_t cap, uint32_t Nsum);
+size_t batch_workspace_size(uint32_t Nsum);
+kr_error tree_batch_launch(const kr_bvh_node* nodes, const uint32_t* parents, const kr_vec3* V, const kr_uvec4* F,
+                           const uint32_t* prim_off, const uint32_t* vert_off, uint32_t K, uint32_t Nsum,
+                           kr_obb3* obbs, kr_mat4* mats, void* workspace, size_t ws_bytes, hipStream_t stream);
+kr_error tree_batch_status_launch(const void* workspace, size_t ws_bytes, uint32_t Nsum, uint32_t* status,
+                                  hipStream_t stream);
 
 }  // namespace obvh

@@ -215,6 +215,67 @@ def obb_tree(nodes, parents, V, F, with_ids: bool = False, stream=None, workspac
     return (obbs, mats, ids) if with_ids else (obbs, mats)
 
 
+def pack_trees(trees):
+    """Concatenate trees for obb_tree_batch.  trees: list of (nodes, parents,
+    V, F) device tensors in the single-tree layout (nodes uint8 (2N-1)*36,
+    parents int32 (2N-1), V float32 (nv, 3), F int32 (N, 4)).  Returns a dict
+    with the batch arrays, the offsets (device int32 and host int64) and the
+    primitive total."""
+    torch = _torch()
+    import numpy as _np
+
+    if not trees:
+        raise ValueError("no trees")
+    prim = _np.zeros(len(trees) + 1, dtype=_np.int64)
+    vert = _np.zeros(len(trees) + 1, dtype=_np.int64)
+    for t, (nd, par, V, F) in enumerate(trees):
+        n = int(F.shape[0])
+        if n < 1 or int(par.numel()) != 2 * n - 1:
+            raise ValueError(f"tree {t}: needs N >= 1 faces and 2N - 1 nodes")
+        prim[t + 1] = prim[t] + n
+        vert[t + 1] = vert[t] + int(V.shape[0])
+    if prim[-1] > 0x7FFFFFFF:
+        raise ValueError("more than 2^31 - 1 primitives in the batch")
+    return {
+        "nodes": torch.cat([nd.reshape(-1) for nd, _, _, _ in trees]),
+        "parents": torch.cat([par.reshape(-1) for _, par, _, _ in trees]),
+        "V": torch.cat([V.reshape(-1, 3) for _, _, V, _ in trees]),
+        "F": torch.cat([F.reshape(-1, 4) for _, _, _, F in trees]),
+        "prim_offsets": torch.from_numpy(prim.astype(_np.int32)).cuda(),
+        "vert_offsets": torch.from_numpy(vert.astype(_np.int32)).cuda(),
+        "prim_offsets_host": prim,
+        "total": int(prim[-1]),
+        "ntrees": len(trees),
+    }
+
+
+def obb_tree_batch(batch, stream=None, workspace=None, out=None, status=None):
+    """obvh_obb_tree_batch over a pack_trees() dict.  Returns (obbs, mats) in
+    the batch's node layout (tree t's nodes from 2 L_t - t, L_t its first
+    primitive).  status: optional int32 tensor receiving the batch's error word."""
+    torch = _torch()
+    total, K = batch["total"], batch["ntrees"]
+    nodes_total = 2 * total - K
+    with _on(stream):
+        if out is None:
+            obbs = torch.empty((nodes_total, 15), dtype=torch.float32, device="cuda")
+            mats = torch.empty((nodes_total, 16), dtype=torch.float32, device="cuda")
+        else:
+            obbs, mats = out
+        need = lib().obvh_obb_tree_batch_workspace_size(K, total)
+        ws = workspace if workspace is not None else _workspace(need)
+    if ws.numel() < need:
+        raise ValueError(f"workspace needs {need} bytes")
+    check(lib().obvh_obb_tree_batch(_ptr(batch["nodes"]), _ptr(batch["parents"]), _ptr(batch["V"]), _ptr(batch["F"]),
+                                    _ptr(batch["prim_offsets"]), _ptr(batch["vert_offsets"]), K, total, _ptr(obbs),
+                                    _ptr(mats), _ptr(ws), ws.numel(), _stream(stream)))
+    if status is not None:
+        if status.dtype != torch.int32 or status.numel() < 1:
+            raise ValueError("status must be an int32 tensor with one element")
+        check(lib().obvh_obb_tree_batch_status_async(_ptr(ws), ws.numel(), total, _ptr(status), _stream(stream)))
+    return obbs, mats
+
+
 def tree_status_message(word: int):
     """Error message of a nonzero transform status word (None for 0)."""
     m = lib().obvh_obb_tree_status_string(int(word) & 0xFFFFFFFF)
