@@ -246,15 +246,20 @@ def serving_leg(args) -> dict:
     t0 = time.perf_counter()
     fit_pool, tree_pool = serve.request_pool()
     print(f"[bench] serving pool ready in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    # warm-up run: every kernel loaded, the allocator's pools grown
+    serve.run_open_loop(serve.Server(hbm_budget=200e9), serve.poisson_schedule(args.serve_rate, 2.0, seed=99),
+                        fit_pool, tree_pool)
     sched = serve.poisson_schedule(args.serve_rate, args.serve_seconds)
-    srv = serve.Server(hbm_budget=200e9)
+    srv = serve.Server(hbm_budget=200e9, request_timeout=1.0)
     w0 = time.perf_counter()
     done = serve.run_open_loop(srv, sched, fit_pool, tree_pool)
     wall = time.perf_counter() - w0
     out = {"open_loop": serve.latency_stats(done)}
     out["open_loop"].update({"rate": args.serve_rate, "seconds": args.serve_seconds,
                              "req_per_s": round(len(done) / wall, 1), "fit_batches": srv.batches,
-                             "mean_fits_per_batch": round(float(np.mean(srv.batch_sizes)), 2)})
+                             "mean_fits_per_batch": round(float(np.mean(srv.batch_sizes)), 2),
+                             "tree_batches": srv.tree_batches, "request_timeout_s": 1.0,
+                             "timeouts": srv.timeouts})
     print(f"[bench] serving open loop: {json.dumps(out['open_loop'])}", file=sys.stderr, flush=True)
     # closed loop: a burst of fits, all arrived at t = 0
     burst = 4096

@@ -135,22 +135,31 @@ def trace_to_numpy(tr) -> np.ndarray:
 
 # ---------------------------------------------------------------- LBVH --
 
-def lbvh_build(V, F, stream=None):
+def lbvh_build(V, F, stream=None, out=None, workspace=None):
     """Device LBVH over (V,3) float32 vertices and (F,4) int32/uint32 faces.
 
     Returns dict(nodes=uint8 tensor (2F-1)*36 B, parents, counts (int32
     tensors, values are u32), costs (float32), mortons (int32, sorted)).
-    """
+    out: optional dict of preallocated tensors with those keys (e.g. views
+    into a batch's arrays)."""
     torch = _torch()
     nf = int(F.shape[0])
     total = 2 * nf - 1
     with _on(stream):
-        nodes = torch.empty(total * NODE_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
-        parents = torch.empty(total, dtype=torch.int32, device="cuda")
-        counts = torch.empty(total, dtype=torch.int32, device="cuda")
-        costs = torch.empty(total, dtype=torch.float32, device="cuda")
-        mortons = torch.empty(nf, dtype=torch.int32, device="cuda")
-        ws = _workspace(lib().obvh_lbvh_workspace_size(nf))
+        if out is not None:
+            nodes, parents, counts, costs, mortons = (out[k] for k in ("nodes", "parents", "counts", "costs",
+                                                                      "mortons"))
+            if (nodes.numel() != total * NODE_DTYPE.itemsize or parents.numel() != total or counts.numel() != total
+                    or costs.numel() != total or mortons.numel() != nf):
+                raise ValueError("lbvh_build: output tensors of the wrong size")
+        else:
+            nodes = torch.empty(total * NODE_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+            parents = torch.empty(total, dtype=torch.int32, device="cuda")
+            counts = torch.empty(total, dtype=torch.int32, device="cuda")
+            costs = torch.empty(total, dtype=torch.float32, device="cuda")
+            mortons = torch.empty(nf, dtype=torch.int32, device="cuda")
+        need = lib().obvh_lbvh_workspace_size(nf)
+        ws = workspace if workspace is not None and workspace.numel() >= need else _workspace(need)
     check(lib().obvh_lbvh_build(_ptr(V), _ptr(F), nf, _ptr(nodes), _ptr(parents), _ptr(counts), _ptr(costs),
                                 _ptr(mortons), _ptr(ws), ws.numel(), _stream(stream)))
     return {"nodes": nodes, "parents": parents, "counts": counts, "costs": costs, "mortons": mortons}

@@ -8,18 +8,26 @@ Two request kinds:
             HBM, still charged to the budget, until ``release(request)``.
 
 Each scheduler iteration
-  1. admits arrived requests in arrival order while their device footprint
+  1. fails requests that waited longer than the request timeout;
+  2. admits arrived requests in arrival order while their device footprint
      fits the HBM budget (inputs, outputs and workspaces stay resident until
      the result is back on the host);
-  2. packs every admitted fit into ONE batched DiTO launch (one gather
+  3. packs every admitted fit into ONE batched DiTO launch (one gather
      kernel reads every set from pinned host memory into the batch's SoA
      array, then obvh_dito_batch, download) on one of two alternating
      streams, so a new batch starts while the previous one still runs;
-  3. starts each admitted tree request on the next of several streams
-     (upload, obvh_lbvh_build, obvh_obb_tree_async, root box download);
-  4. polls completion events and returns results.
+  4. packs every admitted tree request into ONE batched transform on the
+     next of several streams: each mesh is uploaded into the batch's arrays
+     and gets its LBVH built there (obvh_lbvh_build), then
+     obvh_obb_tree_batch transforms all of them at once and the root boxes
+     come back to the host;
+  5. polls completion events and returns results; a request in flight past
+     its timeout is answered with an error at once (its device work still
+     finishes and returns its bytes to the budget).
 Latency is arrival -> result on the host.  The kernels are deterministic, so
-a request's result is the same alone and in any mix.
+a request's result is the same alone and in any mix.  Each request records
+when it was admitted and when its launch was enqueued, so a slow request
+can be attributed to queueing, host-side launch work or the device.
 """
 from __future__ import annotations
 
@@ -48,6 +56,9 @@ class Request:
     result: object = None
     error: str | None = None  # set instead of result when the request failed
     nbytes: int = 0
+    admitted: float = math.nan   # server clock when admitted
+    launched: float = math.nan   # server clock when its launch was enqueued
+    abandoned: bool = False      # answered (timeout) while its device work still ran
 
 
 def fit_bytes(n: int) -> int:
@@ -55,9 +66,12 @@ def fit_bytes(n: int) -> int:
 
 
 def tree_bytes(nv: int, nf: int) -> int:
+    """Device bytes of one tree request in a batch: mesh, tree, boxes and
+    matrices, LBVH scratch, and its share of the batched transform's
+    workspace (the forest copy of its tree plus the transform workspace)."""
     total = 2 * nf - 1
     return (12 * nv + 16 * nf + total * (36 + 4 + 4 + 4 + 60 + 64) + nf * 4
-            + int(lib().obvh_obb_tree_workspace_size(nf, nf - 1)) + int(lib().obvh_lbvh_workspace_size(nf)))
+            + int(lib().obvh_obb_tree_batch_workspace_size(1, nf)) + int(lib().obvh_lbvh_workspace_size(nf)))
 
 
 class Server:
@@ -68,11 +82,14 @@ class Server:
     # 60-69 K with 2^27 (two batches, the first gather waiting for the host
     # to pack half the burst), before the gather's 16-byte reads (77 K).
     def __init__(self, hbm_budget: float = 200e9, tree_streams: int = 4, max_batch_points: int = 1 << 24,
-                 keep_trees: bool = False):
+                 keep_trees: bool = False, request_timeout: float | None = None):
         import torch
 
         self.torch = torch
         self.keep_trees = keep_trees
+        self.request_timeout = request_timeout
+        self.timeouts = 0
+        self.tree_batches = 0
         self.budget = int(hbm_budget)
         self.in_use = 0
         self.max_batch_points = max_batch_points
@@ -82,7 +99,8 @@ class Server:
         self._fs = 0
         self._ts = 0
         self.waiting: collections.deque[Request] = collections.deque()
-        self.inflight: list = []  # (event, callback)
+        self.inflight: list = []  # (event, callback, requests)
+        self._base = (0.0, time.perf_counter())  # (server clock, perf_counter) at the current step
         self.finished: list[Request] = []
         self.batches = 0
         self.batch_sizes: list[int] = []
@@ -97,7 +115,22 @@ class Server:
             raise ValueError(f"request {req.rid} needs {req.nbytes} B, more than the HBM budget")
         self.waiting.append(req)
 
-    def _admit(self) -> tuple[list[Request], list[Request]]:
+    def _expire(self, now: float) -> None:
+        """Fail waiting requests older than the request timeout."""
+        if self.request_timeout is None or not self.waiting:
+            return
+        keep = collections.deque()
+        for r in self.waiting:
+            if now - r.arrival > self.request_timeout:
+                r.error = "timed out waiting for admission"
+                r.done = now
+                self.timeouts += 1
+                self.finished.append(r)
+            else:
+                keep.append(r)
+        self.waiting = keep
+
+    def _admit(self, now: float = math.nan) -> tuple[list[Request], list[Request]]:
         fits, trees, pts = [], [], 0
         while self.waiting:
             r = self.waiting[0]
@@ -112,6 +145,7 @@ class Server:
             else:
                 trees.append(r)
             self.in_use += r.nbytes
+            r.admitted = now
             self.waiting.popleft()
         return fits, trees
 
@@ -159,23 +193,108 @@ class Server:
         def finish(now, fits=fits, host=host, keep=keep):
             res = host.numpy()
             for k, r in enumerate(fits):
+                self.in_use -= r.nbytes
+                if r.abandoned:
+                    continue
                 r.result = res[k].copy()
                 r.done = now
-                self.in_use -= r.nbytes
                 self.finished.append(r)
 
-        self.inflight.append((ev, finish))
+        self.inflight.append((ev, finish, fits))
         self.batches += 1
         self.batch_sizes.append(len(fits))
 
-    def _launch_tree(self, r: Request) -> None:
-        """Mesh -> LBVH -> OBB tree; the host receives the root box, which
-        marks completion.  With ``keep_trees`` the result also holds the
-        device tensors and the request stays charged until ``release``."""
+    def _launch_trees(self, trees: list[Request]) -> None:
+        """Meshes -> LBVHs -> one batched OBB-tree transform; the host
+        receives each root box, which marks completion.  With ``keep_trees``
+        each result also holds its tree's device tensors (views into the
+        batch's arrays) and stays charged until ``release``."""
         torch = self.torch
         self._streams()
         stream = self.tree_streams[self._ts]
         self._ts = (self._ts + 1) % len(self.tree_streams)
+        if len(trees) == 1:  # alone: the single-tree transform, no packing
+            self._launch_tree(trees[0], stream)
+            return
+        nf = [int(r.faces.shape[0]) for r in trees]
+        nv = [int(r.vertices.shape[0]) for r in trees]
+        for r, n in zip(trees, nf):
+            if n < 1 or r.faces.dim() != 2 or r.faces.shape[1] != 4:
+                raise ValueError(f"request {r.rid}: faces must be an (F >= 1, 4) int32 tensor")
+        prim = np.zeros(len(trees) + 1, dtype=np.int64)
+        vert = np.zeros(len(trees) + 1, dtype=np.int64)
+        prim[1:] = np.cumsum(nf)
+        vert[1:] = np.cumsum(nv)
+        total, K = int(prim[-1]), len(trees)
+        nodes_total = 2 * total - K
+        with torch.cuda.stream(stream):
+            V = torch.empty((int(vert[-1]), 3), dtype=torch.float32, device="cuda")
+            F = torch.empty((total, 4), dtype=torch.int32, device="cuda")
+            nodes = torch.empty(nodes_total * NODE_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+            parents = torch.empty(nodes_total, dtype=torch.int32, device="cuda")
+            counts = torch.empty(nodes_total, dtype=torch.int32, device="cuda")
+            costs = torch.empty(nodes_total, dtype=torch.float32, device="cuda")
+            mortons = torch.empty(total, dtype=torch.int32, device="cuda")
+            lws = torch.empty(max(int(lib().obvh_lbvh_workspace_size(max(nf))), 256), dtype=torch.uint8,
+                              device="cuda")
+            offs_h = torch.from_numpy(np.stack([prim, vert]).astype(np.int32)).pin_memory()
+            offs = offs_h.to("cuda", non_blocking=True)
+            spans = []
+            for t, r in enumerate(trees):
+                s0, s1 = 2 * int(prim[t]) - t, 2 * int(prim[t + 1]) - t - 1
+                v0, v1 = int(vert[t]), int(vert[t + 1])
+                f0, f1 = int(prim[t]), int(prim[t + 1])
+                V[v0:v1].copy_(r.vertices, non_blocking=True)
+                F[f0:f1].copy_(r.faces, non_blocking=True)
+                ops.lbvh_build(V[v0:v1], F[f0:f1], stream=stream, workspace=lws,
+                               out={"nodes": nodes[s0 * NODE_DTYPE.itemsize:s1 * NODE_DTYPE.itemsize],
+                                    "parents": parents[s0:s1], "counts": counts[s0:s1], "costs": costs[s0:s1],
+                                    "mortons": mortons[f0:f1]})
+                spans.append((s0, s1))
+            batch = {"nodes": nodes, "parents": parents, "V": V, "F": F, "prim_offsets": offs[0],
+                     "vert_offsets": offs[1], "prim_offsets_host": prim, "total": total, "ntrees": K}
+            status = torch.full((1,), -1, dtype=torch.int32, pin_memory=True)
+            obbs, mats = ops.obb_tree_batch(batch, stream=stream, status=status)
+            roots_idx = torch.tensor([s0 for s0, _ in spans], dtype=torch.int64).pin_memory().to(
+                "cuda", non_blocking=True)
+            roots = torch.empty((K, 15), dtype=torch.float32, pin_memory=True)
+            roots.copy_(obbs.index_select(0, roots_idx), non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(stream)
+        keep = (V, F, nodes, parents, counts, costs, mortons, lws, offs, offs_h, roots_idx)
+        now = self._clock()
+        for r in trees:
+            r.launched = now
+
+        def finish(now, trees=trees, roots=roots, obbs=obbs, mats=mats, keep=keep, stream=stream, status=status,
+                   spans=spans):
+            word = int(status.item())
+            res = roots.numpy()
+            for t, r in enumerate(trees):
+                if r.abandoned:
+                    self.in_use -= r.nbytes
+                    continue
+                r.done = now
+                if word != 0:  # the device found a malformed tree in the batch: fail its requests
+                    r.error = ops.tree_status_message(word)
+                    r.result = None
+                    self.in_use -= r.nbytes
+                else:
+                    r.result = {"root": res[t].copy()}
+                    if self.keep_trees:
+                        s0, s1 = spans[t]
+                        r.result.update({"obbs": obbs[s0:s1], "mats": mats[s0:s1],
+                                         "nodes": keep[2][s0 * NODE_DTYPE.itemsize:s1 * NODE_DTYPE.itemsize],
+                                         "stream": stream})
+                    else:
+                        self.in_use -= r.nbytes
+                self.finished.append(r)
+
+        self.inflight.append((ev, finish, trees))
+        self.tree_batches += 1
+
+    def _launch_tree(self, r: Request, stream) -> None:
+        torch = self.torch
         with torch.cuda.stream(stream):
             V = r.vertices.to("cuda", non_blocking=True)
             F = r.faces.to("cuda", non_blocking=True)
@@ -187,25 +306,27 @@ class Server:
             ev = torch.cuda.Event()
             ev.record(stream)
         keep = (V, F, t)
+        r.launched = self._clock()
 
         def finish(now, r=r, root=root, obbs=obbs, mats=mats, keep=keep, stream=stream, status=status):
             word = int(status.item())
+            if r.abandoned:
+                self.in_use -= r.nbytes
+                return
+            r.done = now
             if word != 0:  # the device found a malformed tree: fail the request
                 r.error = ops.tree_status_message(word)
                 r.result = None
-                r.done = now
                 self.in_use -= r.nbytes
-                self.finished.append(r)
-                return
-            r.result = {"root": root.numpy().copy()}
-            if self.keep_trees:
-                r.result.update({"obbs": obbs, "mats": mats, "nodes": keep[2]["nodes"], "stream": stream})
             else:
-                self.in_use -= r.nbytes
-            r.done = now
+                r.result = {"root": root.numpy().copy()}
+                if self.keep_trees:
+                    r.result.update({"obbs": obbs, "mats": mats, "nodes": keep[2]["nodes"], "stream": stream})
+                else:
+                    self.in_use -= r.nbytes
             self.finished.append(r)
 
-        self.inflight.append((ev, finish))
+        self.inflight.append((ev, finish, [r]))
 
     def release(self, r: Request) -> None:
         """Drop a resident tree (``keep_trees``) and return its bytes to the budget."""
@@ -216,23 +337,42 @@ class Server:
 
     # ------------------------------------------------------------- loop --
     def poll(self, now: float) -> int:
+        if now >= self._base[0]:
+            self._base = (now, time.perf_counter())
         still, n = [], 0
-        for ev, cb in self.inflight:
+        for ev, cb, reqs in self.inflight:
             if ev.query():
-                cb(now)
+                cb(max(now, self._clock()))
                 n += 1
             else:
-                still.append((ev, cb))
+                if self.request_timeout is not None:
+                    for r in reqs:  # answer late requests now; their bytes return when the work ends
+                        if not r.abandoned and now - r.arrival > self.request_timeout:
+                            r.abandoned = True
+                            r.error = "timed out on the device"
+                            r.done = now
+                            self.timeouts += 1
+                            self.finished.append(r)
+                still.append((ev, cb, reqs))
         self.inflight = still
         return n
 
+    def _clock(self) -> float:
+        now, base = self._base
+        return now + (time.perf_counter() - base)
+
     def step(self, now: float) -> None:
-        fits, trees = self._admit()
+        self._base = (now, time.perf_counter())
+        self._expire(now)
+        fits, trees = self._admit(now)
         if fits:
             self._launch_fits(fits)
-        for r in trees:
-            self._launch_tree(r)
-        self.poll(now)
+            t = self._clock()
+            for r in fits:
+                r.launched = t
+        if trees:
+            self._launch_trees(trees)
+        self.poll(self._clock())
 
     def idle(self) -> bool:
         return not self.waiting and not self.inflight
@@ -274,8 +414,25 @@ def poisson_schedule(rate: float, duration: float, fit_share: float = 0.7, seed:
         out.append((t, "fit" if rng.random() < fit_share else "tree", int(rng.integers(1 << 30))))
 
 
-def run_open_loop(server: Server, schedule, fit_pool, tree_pool, timeout: float = 600.0):
-    """Feed the schedule in real time; returns the finished requests."""
+def run_open_loop(server: Server, schedule, fit_pool, tree_pool, timeout: float = 600.0, pause_gc: bool = True):
+    """Feed the schedule in real time; returns the finished requests.
+    pause_gc: no cyclic garbage collection during the run (a full collection
+    over the run's request objects stalls the loop for tens of ms); it runs
+    once afterwards."""
+    import gc
+
+    was = gc.isenabled()
+    if pause_gc:
+        gc.collect()
+        gc.disable()
+    try:
+        return _run_open_loop(server, schedule, fit_pool, tree_pool, timeout)
+    finally:
+        if pause_gc and was:
+            gc.enable()
+
+
+def _run_open_loop(server: Server, schedule, fit_pool, tree_pool, timeout: float):
     t0 = time.perf_counter()
     k = 0
     rid = 0
@@ -302,9 +459,21 @@ def run_open_loop(server: Server, schedule, fit_pool, tree_pool, timeout: float 
 
 
 def latency_stats(done: list[Request]) -> dict:
-    lat = np.array([r.done - r.arrival for r in done]) * 1e3
-    out = {"requests": int(lat.size), "p50_ms": round(float(np.percentile(lat, 50)), 3),
-           "p99_ms": round(float(np.percentile(lat, 99)), 3), "max_ms": round(float(lat.max()), 3)}
+    """p50 / p99 / p99.9 / max latency (ms) of the answered requests, per kind,
+    errors, and the phases of the five slowest requests: queue (arrival ->
+    admission), launch (admission -> launch enqueued) and device (launch ->
+    result on the host, including the poll interval)."""
+    ok = [r for r in done if r.error is None]
+    lat = np.array([r.done - r.arrival for r in ok]) * 1e3
+    out = {"requests": len(done), "errors": len(done) - len(ok),
+           "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+           "p99_9_ms": round(float(np.percentile(lat, 99.9)), 3), "max_ms": round(float(lat.max()), 3)}
+    slow = sorted(ok, key=lambda r: r.done - r.arrival)[-5:][::-1]
+    out["slowest"] = [{"kind": r.kind, "ms": round((r.done - r.arrival) * 1e3, 3),
+                       "queue_ms": round((r.admitted - r.arrival) * 1e3, 3),
+                       "launch_ms": round((r.launched - r.admitted) * 1e3, 3),
+                       "device_ms": round((r.done - r.launched) * 1e3, 3)} for r in slow]
+    done = ok
     for kind in ("fit", "tree"):
         l2 = np.array([r.done - r.arrival for r in done if r.kind == kind]) * 1e3
         if l2.size:
@@ -321,15 +490,19 @@ def main(argv=None) -> int:
     ap.add_argument("--rate", type=float, default=2000.0)
     ap.add_argument("--duration", type=float, default=60.0)
     ap.add_argument("--budget-gb", type=float, default=200.0)
+    ap.add_argument("--timeout", type=float, default=None, help="per-request timeout (s)")
     a = ap.parse_args(argv)
     fit_pool, tree_pool = request_pool()
-    srv = Server(hbm_budget=a.budget_gb * 1e9)
+    run_open_loop(Server(hbm_budget=a.budget_gb * 1e9), poisson_schedule(a.rate, 1.0, seed=99), fit_pool,
+                  tree_pool)  # warm-up: kernels loaded, allocator pools grown
+    srv = Server(hbm_budget=a.budget_gb * 1e9, request_timeout=a.timeout)
     sched = poisson_schedule(a.rate, a.duration)
     t0 = time.perf_counter()
     done = run_open_loop(srv, sched, fit_pool, tree_pool)
     wall = time.perf_counter() - t0
     st = latency_stats(done)
-    st.update({"req_per_s": round(len(done) / wall, 1), "batches": srv.batches,
+    st.update({"req_per_s": round(len(done) / wall, 1), "batches": srv.batches, "tree_batches": srv.tree_batches,
+               "timeouts": srv.timeouts,
                "mean_fits_per_batch": round(float(np.mean(srv.batch_sizes)), 2) if srv.batch_sizes else 0})
     print(json.dumps(st))
     return 0

@@ -87,3 +87,76 @@ def test_results_alone_equal_results_in_the_mix(oracle_lib):
         srv.in_use += r.nbytes
         srv.release(r)
     assert srv.in_use == 0 and all(set(r.result) == {"root"} for r in held)
+
+
+def test_waiting_request_times_out():
+    srv = serve.Server(hbm_budget=serve.fit_bytes(1000) * 2, request_timeout=0.5)
+    r = serve.Request(0, "fit", 0.0, points=_Shape(3, 1000))
+    srv.submit(r)
+    srv.in_use = srv.budget  # no room: the request keeps waiting
+    srv.step(0.4)
+    assert r.error is None and list(srv.waiting) == [r]
+    srv.step(0.6)
+    assert r.error and "timed out" in r.error and not srv.waiting and srv.finished == [r]
+    assert srv.timeouts == 1
+
+
+class _Event:
+    def __init__(self):
+        self.ready = False
+
+    def query(self):
+        return self.ready
+
+
+def test_inflight_request_is_answered_at_its_timeout():
+    """A request past its timeout on the device is answered with an error at
+    once; its bytes return to the budget only when the device work ends."""
+    srv = serve.Server(hbm_budget=1e9, request_timeout=1.0)
+    r = serve.Request(0, "fit", 0.0, points=_Shape(3, 1000))
+    r.nbytes = 100
+    srv.in_use = 100
+    ev, calls = _Event(), []
+
+    def finish(now, r=r):
+        calls.append(now)
+        srv.in_use -= r.nbytes
+        if not r.abandoned:
+            r.result = "box"
+            srv.finished.append(r)
+
+    srv.inflight.append((ev, finish, [r]))
+    srv.poll(0.5)
+    assert r.error is None and srv.finished == []
+    srv.poll(1.5)
+    assert r.abandoned and "timed out" in r.error and srv.finished == [r] and srv.in_use == 100
+    ev.ready = True
+    srv.poll(2.0)
+    assert len(calls) == 1 and calls[0] >= 2.0 and srv.in_use == 0 and r.result is None and srv.finished == [r]
+
+
+@pytest.mark.gpu
+def test_tree_burst_runs_as_one_batch_and_matches_alone():
+    """Tree requests admitted together go through one batched transform;
+    every tree equals the same request served alone (single-tree path)."""
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    print("[serve] tree burst as one batch", flush=True)
+    _, tree_pool = serve.request_pool(seed=5, fits=1, trees=6, tree_range=(500, 30000))
+    srv = serve.Server(keep_trees=True)
+    for i, (V, F) in enumerate(tree_pool):
+        srv.submit(serve.Request(i, "tree", 0.0, vertices=V, faces=F))
+    while not srv.idle():
+        srv.step(0.0)
+    assert srv.tree_batches == 1 and len(srv.finished) == len(tree_pool)
+    for r in srv.finished:
+        alone = serve.Server(keep_trees=True)
+        a = serve.Request(r.rid, "tree", 0.0, vertices=r.vertices, faces=r.faces)
+        alone.submit(a)
+        while not alone.idle():
+            alone.step(0.0)
+        assert r.error is None and a.error is None
+        assert torch.equal(r.result["obbs"], a.result["obbs"]) and torch.equal(r.result["mats"], a.result["mats"])
+        assert np.array_equal(r.result["root"], a.result["root"])
