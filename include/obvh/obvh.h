@@ -130,6 +130,13 @@ typedef struct {
  * error the returned box is all NaN.                                       */
 kr_obb3 kr_cuda_points_obb(const kr_vec3* points, kr_size count, kr_scalar* elapsed_time);
 
+/* The same fit on the host: replaces the reference's CPU kr_points_obb
+ * (src/common/geometry.h:13-14, body geometry.cpp:232-252): DiTO_14<float>
+ * (src/common/algorithm/dito/dito.cpp:478-538) plus the 0.001 extent floor,
+ * single thread; `elapsed_time` (nullable) receives its wall time in ms.
+ * Equal to kr_cuda_points_obb's box for finite input.                      */
+kr_obb3 kr_points_obb(const kr_vec3* points, kr_size count, kr_scalar* elapsed_time);
+
 /* AABB-BVH -> OBB-tree transform.  Replaces kr_cuda_bvh_obb_tree
  * (src/ads/common/cuda/transforms.cuh:52-65, body transforms.cu:966-1059).
  * All pointers are device pointers owned by the caller.  Writes obbs[] and

@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "libobvh.so")
 # every symbol the header declares (checked by tests/test_abi.py)
 EXPORTS = [
     "kr_cuda_points_obb",
+    "kr_points_obb",
     "kr_cuda_bvh_obb_tree",
     "obvh_dito_workspace_size",
     "obvh_dito_soa",
@@ -75,10 +76,19 @@ def lib():
     if not os.path.exists(LIB_PATH):
         raise ObvhError(f"{LIB_PATH} is missing: build it with `python -m obvh_amd.build` "
                         "(or __graft_entry__.build()); there is no CPU fallback")
+    # torch brings its own HIP runtime libraries: have them in the process
+    # before libobvh.so pulls in the system ones, the order under which
+    # device calls from both work in one process
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     vp, sz, u32, u64 = C.c_void_p, C.c_size_t, C.c_uint32, C.c_uint64
     L.kr_cuda_points_obb.argtypes = [vp, u64, C.POINTER(C.c_float)]
     L.kr_cuda_points_obb.restype = KrObb3
+    L.kr_points_obb.argtypes = [vp, u64, C.POINTER(C.c_float)]
+    L.kr_points_obb.restype = KrObb3
     L.kr_cuda_bvh_obb_tree.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, C.c_float, C.c_float, C.c_float,
                                        u32, u32, u32]
     L.kr_cuda_bvh_obb_tree.restype = C.c_char_p

@@ -29,6 +29,12 @@ ARCH = os.environ.get("OBVH_ARCH", "gfx950")
 
 # plugin library name -> value of OBVH_ADS_ATRBVH for ads_plugin.hip
 PLUGINS = {"lbvh_gpu": 0, "atrbvh_gpu": 1}
+# host-only C++ linked into libobvh.so (g++, no FMA contraction)
+HOST_SOURCES = ["cpu_dito.cpp"]
+# the bench integrator plugin (host C++ over libobvh.so)
+BENCH_PLUGIN = "bench"
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Werror",
+              f"-I{os.path.join(ROOT, 'include')}"]
 
 SOURCES = ["abi.hip", "dito.hip", "transform.hip", "lbvh.hip", "collapse.hip", "atrbvh.hip", "traverse.hip", "upload.hip"]
 
@@ -77,6 +83,22 @@ def _compile_plugin(name: str) -> str:
     return obj
 
 
+def gxx() -> str:
+    for cand in (os.environ.get("CXX"), shutil.which("g++")):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("g++ not found: the host C++ sources cannot be built")
+
+
+def _compile_host(src: str, extra=()) -> str:
+    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".host.o")
+    cmd = [gxx(), *HOST_FLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed on {src}:\n{r.stderr}")
+    return obj
+
+
 def _link(objs, out, extra=()) -> None:
     tmp = out + ".tmp"
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, *extra, "-o", tmp]
@@ -90,16 +112,21 @@ def build(verbose: bool = False) -> str:
     """Compile every HIP source, link libobvh.so, then the two intersector
     plugins against it (always rebuilds)."""
     os.makedirs(BUILD, exist_ok=True)
-    with cf.ThreadPoolExecutor(max_workers=min(len(SOURCES) + len(PLUGINS), os.cpu_count() or 4)) as ex:
+    with cf.ThreadPoolExecutor(max_workers=min(len(SOURCES) + len(PLUGINS) + 2, os.cpu_count() or 4)) as ex:
         lib_objs = ex.map(_compile, SOURCES)
+        host_objs = ex.map(_compile_host, HOST_SOURCES)
+        bench_obj = ex.submit(_compile_host, "bench_plugin.cpp", ("-fvisibility=hidden",))
         plugin_objs = {name: ex.submit(_compile_plugin, name) for name in PLUGINS}
-        objs = list(lib_objs)
+        objs = list(lib_objs) + list(host_objs)
         plugin_objs = {name: f.result() for name, f in plugin_objs.items()}
+        bench_obj = bench_obj.result()
     _link(objs, LIB)
     for name, obj in plugin_objs.items():
         _link([obj], plugin_path(name), ("-L" + HERE, "-lobvh", "-Wl,-rpath,$ORIGIN"))
+    _link([bench_obj], plugin_path(BENCH_PLUGIN), ("-L" + HERE, "-lobvh", "-Wl,-rpath,$ORIGIN"))
     if verbose:
-        print(f"built {LIB} and {', '.join(os.path.basename(plugin_path(n)) for n in PLUGINS)}", file=sys.stderr)
+        print(f"built {LIB} and {', '.join(os.path.basename(plugin_path(n)) for n in [*PLUGINS, BENCH_PLUGIN])}",
+              file=sys.stderr)
     return LIB
 
 
