@@ -9,7 +9,7 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libobvh.so")
+LIB_PATH = os.path.join(os.environ.get("OBVH_LIB_DIR") or HERE, "libobvh.so")
 
 # every symbol the header declares (checked by tests/test_abi.py)
 EXPORTS = [

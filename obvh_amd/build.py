@@ -23,8 +23,11 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
-BUILD = os.path.join(HERE, "_build")
-LIB = os.path.join(HERE, "libobvh.so")
+# OBVH_LIB_DIR: where the libraries are built and loaded from (default: the
+# package directory; the sanitizer run uses its own)
+OUT = os.environ.get("OBVH_LIB_DIR") or HERE
+BUILD = os.path.join(OUT, "_build")
+LIB = os.path.join(OUT, "libobvh.so")
 ARCH = os.environ.get("OBVH_ARCH", "gfx950")
 
 # plugin library name -> value of OBVH_ADS_ATRBVH for ads_plugin.hip
@@ -60,9 +63,15 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found: the HIP library cannot be built")
 
 
+# extra flags for every HIP / host compile (set by the sanitizer build)
+HIP_EXTRA: list = []
+HOST_EXTRA: list = []
+HOST_CXX = None
+
+
 def _compile(src: str) -> str:
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    cmd = [hipcc(), *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [hipcc(), *FLAGS, *HIP_EXTRA, "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed on {src}:\n{r.stderr}")
@@ -70,13 +79,13 @@ def _compile(src: str) -> str:
 
 
 def plugin_path(name: str) -> str:
-    return os.path.join(HERE, f"lib{name}.so")
+    return os.path.join(OUT, f"lib{name}.so")
 
 
 def _compile_plugin(name: str) -> str:
     obj = os.path.join(BUILD, f"ads_plugin_{name}.o")
-    cmd = [hipcc(), *FLAGS, f"-DOBVH_ADS_ATRBVH={PLUGINS[name]}", "-c", os.path.join(CSRC, "ads_plugin.hip"),
-           "-o", obj]
+    cmd = [hipcc(), *FLAGS, *HIP_EXTRA, f"-DOBVH_ADS_ATRBVH={PLUGINS[name]}", "-c",
+           os.path.join(CSRC, "ads_plugin.hip"), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed on ads_plugin.hip ({name}):\n{r.stderr}")
@@ -92,7 +101,7 @@ def gxx() -> str:
 
 def _compile_host(src: str, extra=()) -> str:
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".host.o")
-    cmd = [gxx(), *HOST_FLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [HOST_CXX or gxx(), *HOST_FLAGS, *HOST_EXTRA, *extra, "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"g++ failed on {src}:\n{r.stderr}")
@@ -122,8 +131,8 @@ def build(verbose: bool = False) -> str:
         bench_obj = bench_obj.result()
     _link(objs, LIB)
     for name, obj in plugin_objs.items():
-        _link([obj], plugin_path(name), ("-L" + HERE, "-lobvh", "-Wl,-rpath,$ORIGIN"))
-    _link([bench_obj], plugin_path(BENCH_PLUGIN), ("-L" + HERE, "-lobvh", "-Wl,-rpath,$ORIGIN"))
+        _link([obj], plugin_path(name), ("-L" + OUT, "-lobvh", "-Wl,-rpath,$ORIGIN"))
+    _link([bench_obj], plugin_path(BENCH_PLUGIN), ("-L" + OUT, "-lobvh", "-Wl,-rpath,$ORIGIN"))
     if verbose:
         print(f"built {LIB} and {', '.join(os.path.basename(plugin_path(n)) for n in [*PLUGINS, BENCH_PLUGIN])}",
               file=sys.stderr)

@@ -15,7 +15,8 @@ import numpy as np
 from obvh_amd.layouts import ISECT_DTYPE, NODE_DTYPE, RAY_DTYPE, TRACE_DTYPE
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# OBVH_ORACLE_LIB: a differently built copy (the sanitizer run's)
+_LIB_PATH = os.environ.get("OBVH_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 
