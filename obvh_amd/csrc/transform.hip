@@ -47,7 +47,7 @@ namespace obvh {
 
 constexpr uint32_t kInvalid = 0xFFFFFFFFu;
 constexpr uint32_t kChunk0 = 512;     // leaves per workgroup, first climb phase
-constexpr uint32_t kChunk1 = 16384;   // second phase
+constexpr uint32_t kChunk1 = 32768;   // second phase
 
 struct Mat4 {
     float v[16];
@@ -292,7 +292,7 @@ __global__ void batch_holes(TreeWs ws, BatchMap m) {
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k + 1 < m.K; k += gridDim.x * blockDim.x) {
         const uint32_t idx = m.prim_off[k];
         ws.visit[idx] = ((uint64_t)kHoleTag << 32) | 2ull;
-        ws.span[idx] = make_uint2(1u, 0u);  // empty: never listed as a wide node
+        ws.span[idx] = make_uint2(1u, 0u);  // empty: holds no refit tile
     }
 }
 
@@ -301,8 +301,6 @@ __global__ void tree_clear(uint64_t* visit, uint32_t n, TreeWs ws) {
     if (t == 0) {
         *ws.error = 0;
         for (int k = 0; k < kMaxPhases; ++k) ws.nlist[k] = 0;
-        *ws.wide_count = 0;
-        *ws.wide_gap = 0;
     }
     for (uint32_t i = t; i < n; i += gridDim.x * blockDim.x) visit[i] = 0;
 }
@@ -341,15 +339,37 @@ __device__ __forceinline__ void leaf_kdop(f3 v0, f3 v1, f3 v2, const uint32_t vi
 }
 
 // Leaf span of a subtree: first and last leaf position below it.  Bit 31 of
-// .y is set when the positions below are not contiguous (the subtree does
-// not hold every leaf between first and last); leaf positions are < 2^31.
+// .y is set when the subtree does not hold every position from first to
+// last (its leaf count is below the width; the count of such a node is kept
+// beside the span); leaf positions are < 2^31.
 constexpr uint32_t kGap = 0x80000000u;
 __device__ __forceinline__ uint32_t span_hi(uint2 s) { return s.y & ~kGap; }
+__device__ __forceinline__ bool span_gap(uint2 s) { return (s.y & kGap) != 0; }
 __device__ __forceinline__ uint2 leaf_span(uint32_t pos) { return make_uint2(pos, pos); }
-__device__ __forceinline__ uint2 span_join(uint2 a, uint2 b) {
-    const uint32_t ay = a.y & ~kGap, by = b.y & ~kGap;
-    const bool gap = ((a.y | b.y) & kGap) != 0 || !(ay + 1u == b.x || by + 1u == a.x);
-    return make_uint2(min(a.x, b.x), max(ay, by) | (gap ? kGap : 0u));
+// Leaves below a node: the span's width, or the kept count if it has a gap.
+__device__ __forceinline__ uint32_t span_count(uint2 s, uint32_t kept) {
+    return span_gap(s) ? kept : span_hi(s) - s.x + 1u;
+}
+__device__ __forceinline__ uint2 span_join(uint2 a, uint32_t ca, uint2 b, uint32_t cb, uint32_t& c) {
+    const uint32_t lo = min(a.x, b.x), hi = max(span_hi(a), span_hi(b));
+    c = ca + cb;
+    return make_uint2(lo, hi | (hi - lo + 1u != c ? kGap : 0u));
+}
+// Span and leaf count of node c from global memory (the count word is read
+// unconditionally, beside the span, and used only for a gapped span).
+__device__ __forceinline__ uint2 span_load(const TreeWs& ws, uint32_t c, uint32_t I, uint32_t& cnt) {
+    if (c >= I) {
+        cnt = 1;
+        return leaf_span(c - I);
+    }
+    const uint2 s = ws.span[c];
+    const uint32_t kept = ws.count[c];
+    cnt = span_count(s, kept);
+    return s;
+}
+__device__ __forceinline__ void span_store(TreeWs& ws, uint32_t c, uint2 s, uint32_t cnt) {
+    ws.span[c] = s;
+    if (span_gap(s)) ws.count[c] = cnt;
 }
 
 __device__ __forceinline__ void defer(TreeWs& ws, uint32_t phase, uint32_t node, uint32_t I) {
@@ -412,6 +432,7 @@ constexpr uint32_t kTile = kChunk0;
 struct ClimbTile {
     float4 kd[kTile][7];        // k-DOP of internal node lo + s, once completed here
     uint2 span[kTile];
+    uint32_t cnt[kTile];        // leaves below internal node lo + s
     uint32_t lid[kTile][3];     // vertex ids of leaf position lo + s
     uint32_t arrive[kTile];     // arrivals at internal node lo + s from this workgroup
     uint2 kids[kTile];          // children of internal node lo + s
@@ -469,6 +490,7 @@ __global__ __launch_bounds__(kTile) void climb_leaves(const kr_bvh_node* __restr
     if (climbing) {
         uint32_t cur = I + pos;
         uint2 span = leaf_span(pos);
+        uint32_t count = 1;
         uint32_t p = L.par_leaf[t];
         for (uint32_t step = 0; p != kInvalid; ++step) {
             if (p >= I || step >= N) { atomicOr(ws.error, 1u); break; }
@@ -487,7 +509,7 @@ __global__ __launch_bounds__(kTile) void climb_leaves(const kr_bvh_node* __restr
                 pp = L.par_int[s];
             } else {  // parent outside the tile: publish, then global arrival
                 kdop_store(ws.kdop, cur, k);
-                ws.span[cur] = span;
+                span_store(ws, cur, span, count);
                 const uint32_t l = nodes[p].u.children.left, rr = nodes[p].u.children.right;
                 pp = parents[p];
                 const uint64_t old = __hip_atomic_fetch_add(&ws.visit[p], ((uint64_t)tag << 32) | 1ull,
@@ -503,11 +525,13 @@ __global__ __launch_bounds__(kTile) void climb_leaves(const kr_bvh_node* __restr
             // the sibling finished in this workgroup
             Kdop o;
             uint2 so;
+            uint32_t co;
             if (sib >= I) {
                 const uint32_t ss = sib - I - lo;
                 if (sib >= total || ss >= kTile) { atomicOr(ws.error, 2u); break; }
                 tile_leaf_kdop(L, V, ss, o);
                 so = leaf_span(sib - I);
+                co = 1;
             } else if (sib - lo < kTile) {  // completed in this tile: k-DOP in LDS
                 const uint32_t ss = sib - lo;
 #pragma unroll
@@ -517,22 +541,24 @@ __global__ __launch_bounds__(kTile) void climb_leaves(const kr_bvh_node* __restr
                     o.mx[q] = a.z; o.imx[q] = __float_as_uint(a.w);
                 }
                 so = L.span[ss];
+                co = L.cnt[ss];
             } else {
                 kdop_load(ws.kdop, sib, o);
-                so = ws.span[sib];
+                so = span_load(ws, sib, I, co);
             }
             Kdop m;
             if (am_left) merge_left_first(k, o, m); else merge_left_first(o, k, m);
             k = m;
-            span = span_join(span, so);
+            span = span_join(span, count, so, co, count);
             if (s < kTile) {
 #pragma unroll
                 for (int q = 0; q < 7; ++q)
                     L.kd[s][q] = make_float4(k.mn[q], __uint_as_float(k.imn[q]), k.mx[q], __uint_as_float(k.imx[q]));
                 L.span[s] = span;
+                L.cnt[s] = count;
             } else {
                 kdop_store(ws.kdop, p, k);
-                ws.span[p] = span;
+                span_store(ws, p, span, count);
             }
             cur = p;
             p = pp;
@@ -568,37 +594,101 @@ __global__ __launch_bounds__(kTile) void climb_leaves(const kr_bvh_node* __restr
         if (L.arrive[ss] == 2u) kd[j] = L.kd[ss][j - ss * 7];
     }
     if (t < nin && L.arrive[t] == 2u) {
-        ws.span[pos] = L.span[t];
+        span_store(ws, pos, L.span[t], L.cnt[t]);
         ws.visit[pos] = ((uint64_t)tag << 33) | 2ull;
     }
 }
 
-// Phase >= 1 of the bottom-up k-DOP pass.  Workgroup w owns the leaf
-// positions [w*chunk, (w+1)*chunk) and starts from the nodes the previous
-// phase deferred whose first leaf it owns.  At a parent, the first arriving
-// child stops; the second continues if the first was completed in an earlier
-// kernel (visible after the kernel boundary) or by its own workgroup
-// (workgroup-scope acq_rel on the arrival word orders the data); otherwise
-// the parent is deferred to the next phase.  The last phase owns every leaf
-// and never defers.  No device-scope fence is issued.
+// Phase >= 1 of the bottom-up k-DOP pass.  Workgroup w takes the nodes the
+// previous phase deferred whose index lies in [w*chunk, (w+1)*chunk) (both
+// children complete, k-DOPs in global memory), merges them and climbs.  At
+// a parent p:
+//   - one child of p completed in an earlier kernel (its arrival is in p's
+//     arrival word, read a level ahead): the sibling's k-DOP is loaded and
+//     the climb goes on;
+//   - otherwise the arrival is counted in an LDS hash keyed by p.  Each
+//     child first writes its k-DOP and span to its side of p's slot; the
+//     first arrival stops, the second reads the other side and goes on.
+//     Slots still holding one arrival at the end are pairs split between
+//     workgroups: they are folded into the global arrival words, and the
+//     second fold defers p to the next phase;
+//   - when p finds no slot (LDS full), the global arrival word with the
+//     workgroup's tag is used, as at the tile boundaries of phase 0.
+// Every completed node's k-DOP and span go to global memory as they are
+// made (the candidate search and the next phase read them).  The last phase
+// is one workgroup and defers nothing.  Arrivals within the workgroup need
+// workgroup-scope ordering only, on LDS.
+constexpr int kPSlots = 600;   // LDS hash slots (both children's k-DOPs each)
+constexpr int kPProbe = 64;
+constexpr int kPQueue = 1024;  // entries taken from the previous phase's list
+
+struct PhaseTile {
+    float4 kd[kPSlots][2][7];  // side 0: left child, side 1: right child
+    uint2 span[kPSlots][2];
+    uint32_t cnt[kPSlots][2];
+    uint32_t key[kPSlots];
+    uint32_t arrive[kPSlots];
+    uint32_t queue[kPQueue];
+    uint32_t nq;
+};
+
+// Slot of node p, inserted if absent; kInvalid when the probe window is full
+// (keys are never removed, so both children of p get the same answer).
+__device__ __forceinline__ uint32_t phase_slot(PhaseTile& P, uint32_t p) {
+    uint32_t h = __umulhi(p * 2654435761u, (uint32_t)kPSlots);
+    for (int probe = 0; probe < kPProbe; ++probe) {
+        uint32_t key = __hip_atomic_load(&P.key[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (key == kInvalid) key = atomicCAS(&P.key[h], kInvalid, p);
+        if (key == kInvalid || key == p) return h;
+        h = (h + 1 == (uint32_t)kPSlots) ? 0u : h + 1;
+    }
+    return kInvalid;
+}
+
 __global__ __launch_bounds__(1024) void climb_phase(const kr_bvh_node* __restrict__ nodes,
                                                     const uint32_t* __restrict__ parents, TreeWs ws, uint32_t N,
                                                     uint32_t phase, uint32_t chunk) {
-    const uint32_t I = N - 1, total = 2 * N - 1;
+    __shared__ PhaseTile P;
+    const uint32_t I = N - 1, total = 2 * N - 1, t = threadIdx.x;
     const uint32_t tag = make_tag(phase, blockIdx.x);
-    const uint64_t lo = (uint64_t)blockIdx.x * chunk, hi = lo + chunk;  // owned leaf positions
+    const uint64_t lo = (uint64_t)blockIdx.x * chunk, hi = lo + chunk;  // owned node indices
     const uint32_t nin = min(ws.nlist[phase - 1], I);
     const uint32_t* in = ws.list + (size_t)(phase - 1) * (I + 1);
-    auto span_of = [&](uint32_t c) { return c >= I ? leaf_span(c - I) : ws.span[c]; };
-    for (uint32_t r = threadIdx.x; r < nin; r += blockDim.x) {
-        uint32_t cur = in[r];
+    for (uint32_t h = t; h < (uint32_t)kPSlots; h += blockDim.x) {
+        P.key[h] = kInvalid;
+        P.arrive[h] = 0;
+    }
+    if (t == 0) P.nq = 0;
+    __syncthreads();
+    // the owned entries, four independent reads per lane and round
+    for (uint32_t r0 = t; r0 < nin; r0 += 4 * blockDim.x) {
+        uint32_t e[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t r = r0 + j * blockDim.x;
+            e[j] = r < nin ? in[r] : kInvalid;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (e[j] == kInvalid || e[j] < lo || e[j] >= hi) continue;
+            const uint32_t q = atomicAdd(&P.nq, 1u);
+            if (q < (uint32_t)kPQueue) P.queue[q] = e[j];
+        }
+    }
+    __syncthreads();
+    const uint32_t nq = P.nq;
+    const bool queued = nq <= (uint32_t)kPQueue;
+    // more entries than the queue holds: every lane scans the list again
+    for (uint32_t r = t; r < (queued ? nq : nin); r += blockDim.x) {
+        uint32_t cur = queued ? P.queue[r] : in[r];
+        if (!queued && (cur < lo || cur >= hi)) continue;
         if (cur >= I) { atomicOr(ws.error, 1u); continue; }
         const uint32_t l0 = nodes[cur].u.children.left, r0 = nodes[cur].u.children.right;
         if (l0 >= total || r0 >= total) { atomicOr(ws.error, 2u); continue; }
-        uint2 span = span_join(span_of(l0), span_of(r0));
-        if (span.x < lo || span.x >= hi) continue;  // another workgroup owns its first leaf
-        if (span_hi(span) >= hi) { defer(ws, phase, cur, I); continue; }  // wider than this workgroup
         Kdop k;
+        uint32_t count, c0l, c0r;
+        const uint2 s0l = span_load(ws, l0, I, c0l), s0r = span_load(ws, r0, I, c0r);
+        uint2 span = span_join(s0l, c0l, s0r, c0r, count);
         {
             Kdop a, b;
             kdop_load(ws.kdop, l0, a);
@@ -606,48 +696,84 @@ __global__ __launch_bounds__(1024) void climb_phase(const kr_bvh_node* __restric
             merge_left_first(a, b, k);
         }
         kdop_store(ws.kdop, cur, k);
-        ws.span[cur] = span;
-        // One memory round trip per level: the next level's topology (read
-        // only) is fetched a level ahead, and the sibling's k-DOP and span
-        // are read together with the arrival atomic.  That early read is
-        // used only when the sibling was completed by an earlier kernel
-        // (visible since the launch); a sibling completed by this workgroup
-        // in this phase is read again after the acquire.
+        span_store(ws, cur, span, count);
+        // the parent's topology and arrival word are read a level ahead
         uint32_t p = parents[cur];
         uint32_t l = 0, rr = 0, pp = kInvalid;
-        if (p < I) { l = nodes[p].u.children.left; rr = nodes[p].u.children.right; pp = parents[p]; }
+        uint64_t v = 0;
+        if (p < I) { l = nodes[p].u.children.left; rr = nodes[p].u.children.right; pp = parents[p]; v = ws.visit[p]; }
         for (uint32_t step = 0; p != kInvalid; ++step) {
             if (p >= I || step >= N) { atomicOr(ws.error, 1u); break; }
             const bool am_left = (l == cur);
             const uint32_t sib = am_left ? rr : l;
             if (sib >= total || (!am_left && rr != cur)) { atomicOr(ws.error, 2u); break; }
-            Kdop o;
-            kdop_load(ws.kdop, sib, o);
-            uint2 so = span_of(sib);
             uint32_t nl = 0, nr = 0, npp = kInvalid;
-            if (pp < I) { nl = nodes[pp].u.children.left; nr = nodes[pp].u.children.right; npp = parents[pp]; }
-            const uint64_t old = __hip_atomic_fetch_add(&ws.visit[p], ((uint64_t)tag << 32) | 1ull,
-                                                        __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-            const uint32_t cnt = (uint32_t)old, t1 = (uint32_t)(old >> 32);
-            if (cnt == 0) break;  // first arrival: the sibling continues
-            if (cnt != 1) { atomicOr(ws.error, 2u); break; }
-            if (t1 != tag && (t1 >> 24) >= phase) { defer(ws, phase, p, I); break; }
-            if (t1 == tag) {  // completed by this workgroup in this kernel
+            uint64_t nv = 0;
+            if (pp < I) { nl = nodes[pp].u.children.left; nr = nodes[pp].u.children.right; npp = parents[pp]; nv = ws.visit[pp]; }
+            const uint32_t c0 = (uint32_t)v, t0 = (uint32_t)(v >> 32);
+            Kdop o;
+            uint2 so;
+            uint32_t co;
+            if (c0 == 1u && (t0 >> 24) < phase) {  // the sibling finished in an earlier kernel
                 kdop_load(ws.kdop, sib, o);
-                so = span_of(sib);
+                so = span_load(ws, sib, I, co);
+                ws.visit[p] = v + (((uint64_t)tag << 32) | 1ull);
+            } else if (c0 > 1u) {
+                atomicOr(ws.error, 2u);
+                break;
+            } else if (const uint32_t h = phase_slot(P, p); h != kInvalid) {
+                const int side = am_left ? 0 : 1;
+#pragma unroll
+                for (int q = 0; q < 7; ++q)
+                    P.kd[h][side][q] = make_float4(k.mn[q], __uint_as_float(k.imn[q]), k.mx[q], __uint_as_float(k.imx[q]));
+                P.span[h][side] = span;
+                P.cnt[h][side] = count;
+                const uint32_t a = __hip_atomic_fetch_add(&P.arrive[h], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (a == 0u) break;  // first arrival: the sibling continues (or the end fold defers p)
+                if (a != 1u) { atomicOr(ws.error, 2u); break; }
+#pragma unroll
+                for (int q = 0; q < 7; ++q) {
+                    const float4 x = P.kd[h][side ^ 1][q];
+                    o.mn[q] = x.x; o.imn[q] = __float_as_uint(x.y);
+                    o.mx[q] = x.z; o.imx[q] = __float_as_uint(x.w);
+                }
+                so = P.span[h][side ^ 1];
+                co = P.cnt[h][side ^ 1];
+                ws.visit[p] = ((uint64_t)tag << 33) | 2ull;
+            } else {  // no LDS slot: the global arrival word (cur's k-DOP is already stored)
+                const uint64_t old = __hip_atomic_fetch_add(&ws.visit[p], ((uint64_t)tag << 32) | 1ull,
+                                                            __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+                const uint32_t cnt = (uint32_t)old, t1 = (uint32_t)(old >> 32);
+                if (cnt == 0) break;
+                if (cnt != 1) { atomicOr(ws.error, 2u); break; }
+                if (t1 != tag && (t1 >> 24) >= phase) { defer(ws, phase, p, I); break; }
+                kdop_load(ws.kdop, sib, o);
+                so = span_load(ws, sib, I, co);
             }
             Kdop m;
             if (am_left) merge_left_first(k, o, m); else merge_left_first(o, k, m);
             k = m;
-            span = span_join(span, so);
+            span = span_join(span, count, so, co, count);
             kdop_store(ws.kdop, p, k);
-            ws.span[p] = span;
+            span_store(ws, p, span, count);
             cur = p;
             p = pp;
             l = nl;
             rr = nr;
             pp = npp;
+            v = nv;
         }
+    }
+    __syncthreads();
+    // pairs split between workgroups: fold the single arrival into the
+    // global word; the second fold defers the node
+    for (uint32_t h = t; h < (uint32_t)kPSlots; h += blockDim.x) {
+        const uint32_t p = P.key[h];
+        if (p == kInvalid || P.arrive[h] != 1u) continue;
+        const uint64_t old = __hip_atomic_fetch_add(&ws.visit[p], ((uint64_t)tag << 32) | 1ull, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+        if ((uint32_t)old == 1u) defer(ws, phase, p, I);
+        else if ((uint32_t)old != 0u) atomicOr(ws.error, 2u);
     }
 }
 
@@ -945,9 +1071,8 @@ __device__ __forceinline__ Best aabb_best(const Kdop& k) {
 }
 
 // Reference mode over the node's 14 points in list order, for the rare
-// nodes with non-finite or huge extremal points (list kRedoList).  Run by
-// the spare workgroups of list_wide, after candidates: its registers do not
-// add to the distinct-mode search's.
+// nodes with non-finite or huge extremal points (list kRedoList), run by
+// cand_redo.
 constexpr int kRedoList = kMaxPhases - 1;  // the climb's deferral lists use 0 and 1
 
 __device__ __forceinline__ void cand_ref(const kr_vec3* __restrict__ V, TreeWs& ws, uint32_t node) {
@@ -989,10 +1114,44 @@ __device__ __forceinline__ void cand_distinct(const kr_vec3* __restrict__ V, Tre
     store_cand(ws, node, b, degen);
 }
 
-__global__ __launch_bounds__(kCandBlock, 4) void candidates(const kr_vec3* __restrict__ V, TreeWs ws, uint32_t I) {
+__device__ __forceinline__ bool holds(uint2 sp, uint32_t first, uint32_t last) {
+    return sp.x <= first && sp.y >= last && !(sp.y & kGap);
+}
+
+// The ancestors of refit tile `tile` that hold all of its leaves: from the
+// tile's first leaf up to the lowest node whose span covers the tile without
+// a gap, then that node and its ancestors (at most kTChain; the next one is
+// kept for a walk).
+__device__ void tile_chain(const uint32_t* __restrict__ parents, TreeWs& ws, uint32_t N, uint32_t tile) {
+    const uint32_t I = N - 1, lo = tile * kRTile, thi = min(lo + kRTile, N) - 1;
+    uint32_t p = parents[I + lo];
+    for (uint32_t step = 0; p < I && step < N && !holds(ws.span[p], lo, thi); ++step) p = parents[p];
+    uint32_t n = 0;
+    while (p < I && n < kTChain) {
+        ws.tchain[(size_t)tile * kTChain + n] = p;
+        ++n;
+        p = parents[p];
+    }
+    ws.tmeta[tile] = make_uint2(n, p < I ? p : kInvalid);
+}
+
+// Workgroups below `cblocks` find the refit tiles' ancestor chains (their
+// dependent loads run beside the search); the others search one node per
+// lane.
+__global__ __launch_bounds__(kCandBlock, 4) void candidates(const kr_vec3* __restrict__ V,
+                                                            const uint32_t* __restrict__ parents, TreeWs ws,
+                                                            uint32_t N, uint32_t cblocks) {
     __shared__ CandTile T;
-    const uint32_t t = threadIdx.x;
-    const uint32_t base = blockIdx.x * kCandBlock;
+    const uint32_t t = threadIdx.x, I = N - 1;
+    if (blockIdx.x < cblocks) {
+        const uint32_t tile = blockIdx.x * kCandBlock + t;
+        if (tile < (N + kRTile - 1) / kRTile) {
+            if (*ws.error != 0) ws.tmeta[tile] = make_uint2(0u, kInvalid);
+            else tile_chain(parents, ws, N, tile);
+        }
+        return;
+    }
+    const uint32_t base = (blockIdx.x - cblocks) * kCandBlock;
     if (t < 16) T.hist[t] = 0;
     const bool bad = *ws.error != 0;
     __syncthreads();
@@ -1070,63 +1229,15 @@ __global__ __launch_bounds__(kCandBlock, 4) void candidates(const kr_vec3* __res
     else cand_distinct<14>(V, ws, T, s, node, cnt);
 }
 
-// Wide nodes for the refit's tile-wide pass.  Workgroup w lists internal
-// nodes [w * chunk, (w + 1) * chunk): it counts its wide nodes, takes their
-// slots with one global atomic, then writes them (slots from an LDS
-// counter).  A few hundred workgroups keep the same-address atomics few.
-constexpr uint32_t kWideBlocks = 256;
-constexpr uint32_t kRedoBlocks = 8;  // workgroups for the reference-mode candidate search
-__device__ __forceinline__ bool wide_node(uint2 sp, bool& gap_wide) {
-    const bool gap = (sp.y & kGap) != 0;
-    const uint32_t hi = span_hi(sp), first = (sp.x + kRTile - 1) / kRTile * kRTile;
-    gap_wide = gap && hi - sp.x >= kRTile - 1;
-    return !gap && first <= hi && hi - first >= kRTile - 1;
-}
-
-// Workgroups wblocks and above run the candidate search in reference mode
-// on the nodes the distinct-mode search handed back.
-__global__ __launch_bounds__(1024) void list_wide(const kr_vec3* __restrict__ V, TreeWs ws, uint32_t I,
-                                                  uint32_t wblocks) {
-    __shared__ uint32_t cnt, gap, base, off;
-    const uint32_t t = threadIdx.x;
-    if (blockIdx.x >= wblocks) {
-        const uint32_t nredo = min(ws.nlist[kRedoList], I);
-        const uint32_t* redo = ws.list + (size_t)kRedoList * (I + 1);
-        for (uint32_t r = (blockIdx.x - wblocks) * blockDim.x + t; r < nredo; r += (gridDim.x - wblocks) * blockDim.x)
-            cand_ref(V, ws, redo[r]);
-        return;
-    }
-    const uint32_t chunk = (I + wblocks - 1) / wblocks;
-    const uint32_t b0 = blockIdx.x * chunk, b1 = min(b0 + chunk, I);
-    if (t == 0) { cnt = 0; gap = 0; off = 0; }
-    __syncthreads();
-    uint32_t mine = 0;
-    bool g = false;
-    for (uint32_t node = b0 + t; node < b1; node += blockDim.x) {
-        bool gw;
-        mine += wide_node(ws.span[node], gw) ? 1u : 0u;
-        g |= gw;
-    }
-    if (mine != 0) atomicAdd(&cnt, mine);  // LDS
-    if (g) gap = 1;
-    __syncthreads();
-    if (t == 0) {
-        if (cnt != 0) base = atomicAdd(ws.wide_count, cnt);
-        if (gap != 0 && __hip_atomic_load(ws.wide_gap, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
-            atomicOr(ws.wide_gap, 1u);
-    }
-    __syncthreads();
-    if (cnt == 0) return;  // uniform
-    for (uint32_t node = b0 + t; node < b1; node += blockDim.x) {
-        const uint2 sp = ws.span[node];
-        bool gw;
-        if (!wide_node(sp, gw)) continue;
-        const uint32_t slot = base + atomicAdd(&off, 1u);
-        if (slot < kWideMax) {
-            ws.wide_span[slot] = sp;
-            ws.wide_node[slot] = node;
-        }
-    }
+// The candidate search in reference mode on the nodes the distinct-mode
+// search handed back (a separate launch after candidates: its registers do
+// not add to the distinct-mode search's).
+constexpr uint32_t kRedoBlocks = 8;
+__global__ __launch_bounds__(1024) void cand_redo(const kr_vec3* __restrict__ V, TreeWs ws, uint32_t I) {
+    const uint32_t nredo = min(ws.nlist[kRedoList], I);
+    const uint32_t* redo = ws.list + (size_t)kRedoList * (I + 1);
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < nredo; r += gridDim.x * blockDim.x)
+        cand_ref(V, ws, redo[r]);
 }
 
 // ------------------------------------------------------------ refit --
@@ -1148,22 +1259,20 @@ __global__ __launch_bounds__(1024) void list_wide(const kr_vec3* __restrict__ V,
 //      vertices instead of 3 per triangle.
 // "Holds all leaves of a range" is read from the span: its first and last
 // leaf cover the range and the leaves below are contiguous.  LDS results are
-// folded into global memory with filtered atomics at the end.  Without the
+// folded into global memory at the end.  Without the
 // tile-wide pass (too many distinct vertices, or no tile-common node) tier B
 // continues to the root.
 // The walks of tiers A and B read the parent, span and axes of nodes in the
 // tile's own index range [lo, lo + 512) from an LDS copy made at the start
 // (a Karras tree numbers every node below the tile-common node inside that
 // range; other nodes are read from global memory).  The ancestors for tier C
-// are the wide nodes whose span holds the tile, found by one parallel scan of
-// the wide-node list instead of a walk up the parent links (which would be
-// ~30 dependent loads deep); when the list cannot describe them (a spanning
-// node with non-contiguous leaves, a list overflow, the short last tile) wave
-// 0 walks the parent links.
+// (the tile-common node and up to kTChain - 1 of its ancestors) were listed
+// per tile by the candidate kernel's leading workgroups, so no workgroup
+// walks the ~10-30 dependent parent loads itself; ancestors beyond the list
+// (very deep trees) are walked by every wave with its lanes' triangles.
 
 constexpr int kRHash = 256;      // LDS hash for ancestors outside the tile's index range
 constexpr int kVSet = 2048;      // distinct vertex ids per tile (hash set)
-constexpr int kChainMax = 128;   // ancestors handled by the tile-wide pass per batch
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 constexpr int32_t kKeyHi = 0x7FFFFFFF, kKeyLo = (int32_t)0x80000000;
 
@@ -1183,7 +1292,7 @@ struct RefitTile {
         RefitNodes nd;           // until tiers A and B end
         float upos[kVSet][3];    // then: the distinct vertices' positions, compacted
     };
-    uint32_t chain[kChainMax];   // tile-common node and its ancestors
+    uint32_t chain[kTChain];     // tile-common node and its ancestors
     uint32_t nchain, nuniq, overflow, rest;
 };
 
@@ -1210,11 +1319,14 @@ __device__ __forceinline__ int32_t okey(float f) {
     return i ^ (int32_t)((uint32_t)(i >> 31) >> 1);
 }
 
+// Unconditional: reading the current keys first to skip atomics made every
+// fold wait for a load of a line other workgroups update (about a fifth of
+// the refit's time on a 2M-leaf tree).
 __device__ __forceinline__ void fold_global(int32_t* rk, const int32_t v[6]) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        if (v[a] < rk[a]) atomicMin(&rk[a], v[a]);
-        if (v[3 + a] > rk[3 + a]) atomicMax(&rk[3 + a], v[3 + a]);
+        atomicMin(&rk[a], v[a]);
+        atomicMax(&rk[3 + a], v[3 + a]);
     }
 }
 
@@ -1282,16 +1394,12 @@ __device__ __forceinline__ void wave_keys(int32_t k[6]) {
     }
 }
 
-__device__ __forceinline__ bool holds(uint2 sp, uint32_t first, uint32_t last) {
-    return sp.x <= first && sp.y >= last && !(sp.y & kGap);
-}
-
 __global__ __launch_bounds__(kRTile) void refit(const kr_bvh_node* __restrict__ nodes,
                                                const uint32_t* __restrict__ parents,
                                                const kr_vec3* __restrict__ V,
                                                const kr_uvec4* __restrict__ F, TreeWs ws, uint32_t N) {
     __shared__ RefitTile R;
-    __shared__ uint32_t bad, listed;
+    __shared__ uint32_t bad;
     const uint32_t I = N - 1, t = threadIdx.x;
     const uint32_t lo = blockIdx.x * kRTile;
     const uint32_t thi = min(lo + kRTile, N) - 1;  // last leaf position of the tile
@@ -1314,27 +1422,15 @@ __global__ __launch_bounds__(kRTile) void refit(const kr_bvh_node* __restrict__ 
     }
     if (t == 0) {
         bad = *ws.error;  // one read per workgroup: the exit is uniform
-        // the wide-node list describes this tile's ancestors: a whole tile,
-        // no spanning node with a gap, every wide node listed
-        const uint32_t wc = *ws.wide_count;
-        listed = (thi - lo + 1 == kRTile && *ws.wide_gap == 0 && wc <= kWideMax) ? wc + 1u : 0u;
-        R.nchain = 0;
+        const uint2 m = ws.tmeta[blockIdx.x];
+        R.nchain = min(m.x, (uint32_t)kTChain);
+        R.rest = m.y;
         R.nuniq = 0;
         R.overflow = 0;
-        R.rest = kInvalid;
     }
+    if (t < kTChain) R.chain[t] = ws.tchain[(size_t)blockIdx.x * kTChain + t];  // entries past the count unused
     __syncthreads();
     if (bad != 0) return;
-    const uint32_t nwide = listed;  // wide nodes + 1, or 0: walk the parent links
-    if (nwide != 0) {
-        for (uint32_t i = t; i + 1 < nwide; i += kRTile) {
-            const uint2 sp = ws.wide_span[i];
-            if (sp.x <= lo && sp.y >= thi) {
-                const uint32_t slot = atomicAdd(&R.nchain, 1u);
-                if (slot < (uint32_t)kChainMax) R.chain[slot] = ws.wide_node[i];
-            }
-        }
-    }
 
     const uint32_t pos = lo + t;
     const uint32_t wlo = pos - (uint32_t)lane_id();
@@ -1366,11 +1462,9 @@ __global__ __launch_bounds__(kRTile) void refit(const kr_bvh_node* __restrict__ 
         }
     }
     __syncthreads();
-    const bool tile_pass = R.overflow == 0;
-    // the list held more ancestors than one batch: walk the parent links
-    const bool use_list = nwide != 0 && R.nchain <= (uint32_t)kChainMax;
-    __syncthreads();  // every lane has read nchain before wave 0 may reset it
-    if (!use_list && t == 0) R.nchain = 0;
+    // tier C needs the tile-common node (chain[0]) and a vertex set that fit
+    const bool tile_pass = R.overflow == 0 && R.nchain != 0;
+    const uint32_t q0 = tile_pass ? R.chain[0] : kInvalid;
 
     if (wlo < N) {  // wave-uniform
         // tier A
@@ -1402,8 +1496,8 @@ __global__ __launch_bounds__(kRTile) void refit(const kr_bvh_node* __restrict__ 
                 uint32_t qp;
                 uint2 sp;
                 Cand c;
+                if (q == q0) break;  // tier C takes q and above
                 refit_node(R, ws, parents, lo, q, qp, sp, c);
-                if (tile_pass && holds(sp, lo, thi)) break;  // tier C takes q and above
                 if (!c.degen) {
                     int32_t k[6];
                     if (valid) {
@@ -1416,20 +1510,6 @@ __global__ __launch_bounds__(kRTile) void refit(const kr_bvh_node* __restrict__ 
                     if (lane_id() == 0) fold_tile(R, ws.range, lo, q, k);
                 }
                 q = __builtin_amdgcn_readfirstlane(qp);
-            }
-            // wave 0 stopped at the tile-common node: without the list,
-            // record it and its ancestors
-            if (!use_list && tile_pass && t < kWave && q != kInvalid && q < I) {
-                uint32_t n = 0;
-                while (q != kInvalid && q < I && n < (uint32_t)kChainMax) {
-                    if (lane_id() == 0) R.chain[n] = q;
-                    ++n;
-                    q = __builtin_amdgcn_readfirstlane(parents[q]);
-                }
-                if (lane_id() == 0) {
-                    R.nchain = n;
-                    R.rest = (q != kInvalid && q < I) ? q : kInvalid;
-                }
             }
         }
     }
@@ -1532,7 +1612,7 @@ __global__ __launch_bounds__(kRTile) void refit(const kr_bvh_node* __restrict__ 
         const uint32_t node = lo + t;
         const uint2 sp = ws.span[node];
         int32_t* rk = &ws.range[(size_t)node * 6];
-        const bool chain_node = tile_pass && holds(sp, lo, thi);
+        const bool chain_node = node == q0;
         if (sp.x >= lo && span_hi(sp) <= thi && !chain_node) {
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
@@ -1635,15 +1715,15 @@ size_t tree_carve(TreeWs* ws, void* base, size_t cap, uint32_t N) {
     ws->visit = c.take<uint64_t>(I + 1);
     ws->error = c.take<uint32_t>(8);
     ws->nlist = ws->error ? ws->error + 2 : nullptr;  // error[2 .. 2 + kMaxPhases)
-    ws->wide_count = ws->error ? ws->error + 6 : nullptr;
-    ws->wide_gap = ws->error ? ws->error + 7 : nullptr;
     ws->list = c.take<uint32_t>((size_t)kMaxPhases * (I + 1));
     ws->span = c.take<uint2>(total);
+    ws->count = c.take<uint32_t>(I + 1);
     ws->kdop = c.take<float4>(total * 7);
     ws->cand = c.take<Cand>(I + 1);
     ws->range = c.take<int32_t>((I + 1) * 6);
-    ws->wide_span = c.take<uint2>(kWideMax);
-    ws->wide_node = c.take<uint32_t>(kWideMax);
+    const size_t tiles = ((size_t)N + kRTile - 1) / kRTile;
+    ws->tchain = c.take<uint32_t>(tiles * kTChain);
+    ws->tmeta = c.take<uint2>(tiles);
     return c.used;
 }
 
@@ -1718,9 +1798,10 @@ kr_error tree_launch(const kr_bvh_node* nodes, const uint32_t* parents, const kr
         }
     }
     if (I > 0) {
-        hipLaunchKernelGGL(candidates, blocks(I, kCandBlock), dim3(kCandBlock), 0, stream, V, ws, I);
-        const uint32_t wb = min(kWideBlocks, (I + 1023) / 1024);
-        hipLaunchKernelGGL(list_wide, dim3(wb + kRedoBlocks), dim3(1024), 0, stream, V, ws, I, wb);
+        const uint32_t cb = (((N + kRTile - 1) / kRTile) + kCandBlock - 1) / kCandBlock;
+        hipLaunchKernelGGL(candidates, dim3(cb + (I + kCandBlock - 1) / kCandBlock), dim3(kCandBlock), 0, stream, V,
+                           parents, ws, N, cb);
+        hipLaunchKernelGGL(cand_redo, dim3(kRedoBlocks), dim3(1024), 0, stream, V, ws, I);
         hipLaunchKernelGGL(refit, blocks(N, kRTile), dim3(kRTile), 0, stream, nodes, parents, V, F, ws, N);
         hipLaunchKernelGGL(finalize, blocks(I, kStage), dim3(kStage), 0, stream, ws, obbs, mats, I, bm);
     }

@@ -19,26 +19,25 @@ struct TreeWs {
     uint64_t* visit;   // per internal node: arrivals (low word) + sum of arrival tags (high word)
     uint32_t* nlist;   // per climb phase: number of deferred nodes
     uint32_t* list;    // per climb phase: nodes deferred to the next phase
-    uint2* span;       // per node: first and last leaf position below it (bit 31 of .y: not contiguous)
+    uint2* span;       // per node: first and last leaf position below it (bit 31 of .y: the
+                       // leaves below are not every position from first to last)
+    uint32_t* count;   // per internal node with bit 31 of its span set: leaves below it
     uint32_t* error;   // 1: parent index out of range / too deep, 2: parent/child mismatch,
                        // 4: face id out of range, 8: a node did not complete
     float4* kdop;      // 7 per node: {min, min vertex id, max, max vertex id}
     Cand* cand;        // per internal node
     int32_t* range;    // 6 per internal node: order-preserving keys, min b0..b2, max b0..b2
-    // Wide nodes: internal nodes whose contiguous leaf span holds at least
-    // one whole aligned refit tile, listed by the list_wide kernel (span and
-    // node id; at most kWideMax).  wide_count counts every such node (it may
-    // exceed the list), wide_gap is nonzero when a node spanning a tile or
-    // more has non-contiguous leaves (the list then does not describe the
-    // ancestors and the refit walks parent links instead).
-    uint2* wide_span;
-    uint32_t* wide_node;
-    uint32_t* wide_count;
-    uint32_t* wide_gap;
+    // Per refit tile (kRTile leaf positions): the lowest node holding all of
+    // the tile's leaves (span covers it, no gap) and its ancestors, at most
+    // kTChain of them (tchain), and {count, next ancestor beyond them or
+    // kInvalid} (tmeta).  Written by the leading workgroups of the candidate
+    // kernel, read by the refit.
+    uint32_t* tchain;
+    uint2* tmeta;
 };
-constexpr uint32_t kWideMax = 16384;
 constexpr uint32_t kRTile = 512;  // leaves per refit workgroup (tiles aligned to multiples of it)
-static_assert(2 + kMaxPhases <= 6, "error words 6 and 7 hold the wide-node count and gap flag");
+constexpr uint32_t kTChain = 64;
+static_assert(2 + kMaxPhases <= 8, "error words 2 .. 2 + kMaxPhases hold the phase list counts");
 
 size_t tree_carve(TreeWs* ws, void* base, size_t cap, uint32_t N);
 struct BatchMap;

@@ -261,13 +261,12 @@ def test_obb_tree_random_topology(torch, oracle_lib, n):
 @pytest.mark.parametrize("n,spine", [(30000, 0), (81920, 512), (81920, 384), (40000, 16)])
 def test_obb_tree_contiguous_topology(torch, oracle_lib, n, spine):
     """Trees whose subtrees hold contiguous leaf runs but whose internal nodes
-    are numbered breadth-first (not Karras): the refit finds each tile's
-    ancestors in the wide-node list while its LDS copy of the tile's index
-    range mostly misses.  spine=512: the leftmost tile has 127 ancestors
-    holding it, within one batch of 128; spine=384: 171, more than one
-    batch, so those tiles walk the parent links instead.  spine=16: a
-    2000-deep spine, about 1200 nodes spanning more than a 16384-leaf chunk,
-    so the top-of-tree climb phases walk long chains."""
+    are numbered breadth-first (not Karras): the refit's LDS copy of the
+    tile's index range mostly misses.  spine=512: the leftmost tile has 127
+    ancestors holding it, more than the 64 listed per tile, so the rest are
+    walked by every wave; spine=384: 171.  spine=16: a 2000-deep spine
+    whose nodes mostly span more than one 32768-leaf chunk, so the
+    top-of-tree climb phases walk long chains."""
     say(f"[obb_tree] contiguous topology n={n} spine={spine}")
     V, F = synth.hairball(tubes=n // 40 + 1, seed=7)
     F = F[:n]
