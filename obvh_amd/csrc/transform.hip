@@ -1051,13 +1051,16 @@ __device__ __forceinline__ bool sane_points(const f3* P, int n) {
     return ex <= ((127u + 60u) << 23);
 }
 
-__device__ __forceinline__ void store_cand(TreeWs& ws, uint32_t node, const Best& b, int degen) {
+// aq: half area of the node's AABB (the k-DOP's first three slabs), kept
+// so that finalize reads the k-DOP only where the AABB wins.
+__device__ __forceinline__ void store_cand(TreeWs& ws, uint32_t node, const Best& b, int degen, float aq) {
     Cand c;
     c.b[0] = b.b0.x; c.b[1] = b.b0.y; c.b[2] = b.b0.z;
     c.b[3] = b.b1.x; c.b[4] = b.b1.y; c.b[5] = b.b1.z;
     c.b[6] = b.b2.x; c.b[7] = b.b2.y; c.b[8] = b.b2.z;
     c.degen = degen;
-    c.pad[0] = c.pad[1] = 0;
+    c.aq = aq;
+    c.pad = 0;
     ws.cand[node] = c;
     int32_t* rk = &ws.range[(size_t)node * 6];
     rk[0] = f2ord(b.dmin.x); rk[1] = f2ord(b.dmin.y); rk[2] = f2ord(b.dmin.z);
@@ -1089,10 +1092,11 @@ __device__ __forceinline__ void cand_ref(const kr_vec3* __restrict__ V, TreeWs& 
     for (int s = 1; s < 7; ++s)
         if (at == s) { p0 = Q[2 * s]; p1 = Q[2 * s + 1]; }
     Best b = aabb_best(k);
+    const float aq = b.q;
     int degen = 0;
     const Pts<14, true> L{Q, sane_points(Q, 14)};
     cand_search(L, p0, p1, b, degen);
-    store_cand(ws, node, b, degen);
+    store_cand(ws, node, b, degen, aq);
 }
 
 // Distinct-mode search of one node with the list padded to NP points.
@@ -1106,12 +1110,13 @@ __device__ __forceinline__ void cand_distinct(const kr_vec3* __restrict__ V, Tre
     const float4* kd = &ws.kdop[(size_t)node * 7];
     const float4 s0 = kd[0], s1 = kd[1], s2 = kd[2];
     const f3 alen = f3{s0.z - s0.x, s1.z - s1.x, s2.z - s2.x};
-    Best b = {quality(alen), f3{1, 0, 0}, f3{0, 1, 0}, f3{0, 0, 1}, f3{s0.x, s1.x, s2.x}, f3{s0.z, s1.z, s2.z}};
+    const float aq = quality(alen);
+    Best b = {aq, f3{1, 0, 0}, f3{0, 1, 0}, f3{0, 0, 1}, f3{s0.x, s1.x, s2.x}, f3{s0.z, s1.z, s2.z}};
     const f3 p0 = ldv(V, T.pair[s][0]), p1 = ldv(V, T.pair[s][1]);
     int degen = 0;
     const Pts<NP, false> L{P, true};
     cand_search(L, p0, p1, b, degen);
-    store_cand(ws, node, b, degen);
+    store_cand(ws, node, b, degen, aq);
 }
 
 __device__ __forceinline__ bool holds(uint2 sp, uint32_t first, uint32_t last) {
@@ -1645,27 +1650,25 @@ __global__ __launch_bounds__(kStage) void finalize(TreeWs ws, kr_obb3* obbs, kr_
         if (*ws.error != 0 || !complete(vw) || hole(vw)) {
             stage_nan(st, threadIdx.x);
         } else {
-            const float4* kd = &ws.kdop[(size_t)node * 7];
-            const float4 s0 = kd[0], s1 = kd[1], s2 = kd[2];
-            const f3 amn = f3{s0.x, s1.x, s2.x}, amx = f3{s0.z, s1.z, s2.z};
-            const f3 alen = sub3(amx, amn);
             const Cand c = ws.cand[node];
             const int32_t* rk = &ws.range[(size_t)node * 6];
             const f3 rmn = f3{ord2f(rk[0]), ord2f(rk[1]), ord2f(rk[2])};
             const f3 rmx = f3{ord2f(rk[3]), ord2f(rk[4]), ord2f(rk[5])};
             f3 v0, v1, v2, mn, mx;
-            if (!c.degen && quality(sub3(rmx, rmn)) < quality(alen)) {
+            if (!c.degen && quality(sub3(rmx, rmn)) < c.aq) {
                 v0 = f3{c.b[0], c.b[1], c.b[2]};
                 v1 = f3{c.b[3], c.b[4], c.b[5]};
                 v2 = f3{c.b[6], c.b[7], c.b[8]};
                 mn = rmn;
                 mx = rmx;
-            } else {
+            } else {  // the AABB: the k-DOP's first three slabs
+                const float4* kd = &ws.kdop[(size_t)node * 7];
+                const float4 s0 = kd[0], s1 = kd[1], s2 = kd[2];
                 v0 = f3{1, 0, 0};
                 v1 = f3{0, 1, 0};
                 v2 = f3{0, 0, 1};
-                mn = amn;
-                mx = amx;
+                mn = f3{s0.x, s1.x, s2.x};
+                mx = f3{s0.z, s1.z, s2.z};
             }
             const f3 mid = vm::mul1(add3(mn, mx), 0.5f);
             const f3 ext = vm::vmax3(vm::mul1(sub3(mx, mn), 0.5f), f3{0.001f, 0.001f, 0.001f});

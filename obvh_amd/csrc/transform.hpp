@@ -10,7 +10,8 @@ namespace obvh {
 struct Cand {
     float b[9];
     int32_t degen;  // 1: the node's box is its AABB
-    int32_t pad[2];
+    float aq;       // half area of the node's AABB
+    int32_t pad;
 };
 
 constexpr int kMaxPhases = 4;
