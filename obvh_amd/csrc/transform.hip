@@ -372,7 +372,19 @@ __device__ __forceinline__ void span_store(TreeWs& ws, uint32_t c, uint2 s, uint
     if (span_gap(s)) ws.count[c] = cnt;
 }
 
+// Node lists (ws.list, counts in ws.nlist): 0 and 1 hold the nodes a climb
+// phase defers to the next, kUpperList the nodes completed after the tile
+// climb, kRedoList the nodes the candidate search hands to reference mode.
+constexpr int kUpperList = 2;
+constexpr int kRedoList = 3;
+static_assert(kRedoList < kMaxPhases, "one list per slot");
+
+// A deferred node is marked in bit 63 of its arrival word: its count already
+// reads 2 (both children arrived), but its k-DOP is made by the next phase,
+// so the candidate launch that runs beside the later phases must skip it.
+constexpr uint64_t kDeferred = 1ull << 63;
 __device__ __forceinline__ void defer(TreeWs& ws, uint32_t phase, uint32_t node, uint32_t I) {
+    atomicOr(reinterpret_cast<unsigned long long*>(&ws.visit[node]), (unsigned long long)kDeferred);
     const uint32_t slot = atomicAdd(&ws.nlist[phase], 1u);
     if (slot < I) ws.list[(size_t)phase * (I + 1) + slot] = node;
 }
@@ -629,7 +641,8 @@ struct PhaseTile {
     uint32_t key[kPSlots];
     uint32_t arrive[kPSlots];
     uint32_t queue[kPQueue];
-    uint32_t nq;
+    uint32_t done[kPQueue];    // nodes completed here, for the second candidate launch
+    uint32_t nq, ndone, dbase;
 };
 
 // Slot of node p, inserted if absent; kInvalid when the probe window is full
@@ -658,7 +671,20 @@ __global__ __launch_bounds__(1024) void climb_phase(const kr_bvh_node* __restric
         P.key[h] = kInvalid;
         P.arrive[h] = 0;
     }
-    if (t == 0) P.nq = 0;
+    if (t == 0) {
+        P.nq = 0;
+        P.ndone = 0;
+    }
+    const bool last = gridDim.x == 1;
+    auto completed = [&](uint32_t node) {
+        const uint32_t d = atomicAdd(&P.ndone, 1u);
+        if (d < (uint32_t)kPQueue) {
+            P.done[d] = node;
+        } else {
+            const uint32_t slot = atomicAdd(&ws.nlist[kUpperList], 1u);
+            if (slot < I) ws.list[(size_t)kUpperList * (I + 1) + slot] = node;
+        }
+    };
     __syncthreads();
     // the owned entries, four independent reads per lane and round
     for (uint32_t r0 = t; r0 < nin; r0 += 4 * blockDim.x) {
@@ -697,6 +723,7 @@ __global__ __launch_bounds__(1024) void climb_phase(const kr_bvh_node* __restric
         }
         kdop_store(ws.kdop, cur, k);
         span_store(ws, cur, span, count);
+        completed(cur);
         // the parent's topology and arrival word are read a level ahead
         uint32_t p = parents[cur];
         uint32_t l = 0, rr = 0, pp = kInvalid;
@@ -746,7 +773,11 @@ __global__ __launch_bounds__(1024) void climb_phase(const kr_bvh_node* __restric
                 const uint32_t cnt = (uint32_t)old, t1 = (uint32_t)(old >> 32);
                 if (cnt == 0) break;
                 if (cnt != 1) { atomicOr(ws.error, 2u); break; }
-                if (t1 != tag && (t1 >> 24) >= phase) { defer(ws, phase, p, I); break; }
+                if (t1 != tag && (t1 >> 24) >= phase) {  // the other child finished in another workgroup
+                    if (last) atomicOr(ws.error, 2u);
+                    else defer(ws, phase, p, I);
+                    break;
+                }
                 kdop_load(ws.kdop, sib, o);
                 so = span_load(ws, sib, I, co);
             }
@@ -756,6 +787,7 @@ __global__ __launch_bounds__(1024) void climb_phase(const kr_bvh_node* __restric
             span = span_join(span, count, so, co, count);
             kdop_store(ws.kdop, p, k);
             span_store(ws, p, span, count);
+            completed(p);
             cur = p;
             p = pp;
             l = nl;
@@ -772,9 +804,15 @@ __global__ __launch_bounds__(1024) void climb_phase(const kr_bvh_node* __restric
         if (p == kInvalid || P.arrive[h] != 1u) continue;
         const uint64_t old = __hip_atomic_fetch_add(&ws.visit[p], ((uint64_t)tag << 32) | 1ull, __ATOMIC_RELAXED,
                                                     __HIP_MEMORY_SCOPE_AGENT);
-        if ((uint32_t)old == 1u) defer(ws, phase, p, I);
+        if ((uint32_t)old == 1u && !last) defer(ws, phase, p, I);
         else if ((uint32_t)old != 0u) atomicOr(ws.error, 2u);
     }
+    // the completed nodes go to the upper list in one block
+    if (t == 0) P.dbase = atomicAdd(&ws.nlist[kUpperList], min(P.ndone, (uint32_t)kPQueue));
+    __syncthreads();
+    const uint32_t nd = min(P.ndone, (uint32_t)kPQueue);
+    for (uint32_t i = t; i < nd; i += blockDim.x)
+        if (P.dbase + i < I) ws.list[(size_t)kUpperList * (I + 1) + P.dbase + i] = P.done[i];
 }
 
 // ------------------------------------------------------- candidates --
@@ -1029,6 +1067,7 @@ struct CandTile {
     uint32_t pair[kCandBlock][2];   // vertex ids of the farthest slab pair
     uint8_t cnt[kCandBlock];        // number of distinct ids; 0 = nothing to search; 15 = reference mode
     uint16_t order[kCandBlock];
+    uint32_t node[kCandBlock];
     uint32_t hist[16];
 };
 
@@ -1076,7 +1115,6 @@ __device__ __forceinline__ Best aabb_best(const Kdop& k) {
 // Reference mode over the node's 14 points in list order, for the rare
 // nodes with non-finite or huge extremal points (list kRedoList), run by
 // cand_redo.
-constexpr int kRedoList = kMaxPhases - 1;  // the climb's deferral lists use 0 and 1
 
 __device__ __forceinline__ void cand_ref(const kr_vec3* __restrict__ V, TreeWs& ws, uint32_t node) {
     Kdop k;
@@ -1140,39 +1178,24 @@ __device__ void tile_chain(const uint32_t* __restrict__ parents, TreeWs& ws, uin
     ws.tmeta[tile] = make_uint2(n, p < I ? p : kInvalid);
 }
 
-// Workgroups below `cblocks` find the refit tiles' ancestor chains (their
-// dependent loads run beside the search); the others search one node per
-// lane.
-__global__ __launch_bounds__(kCandBlock, 4) void candidates(const kr_vec3* __restrict__ V,
-                                                            const uint32_t* __restrict__ parents, TreeWs ws,
-                                                            uint32_t N, uint32_t cblocks) {
-    __shared__ CandTile T;
-    const uint32_t t = threadIdx.x, I = N - 1;
-    if (blockIdx.x < cblocks) {
-        const uint32_t tile = blockIdx.x * kCandBlock + t;
-        if (tile < (N + kRTile - 1) / kRTile) {
-            if (*ws.error != 0) ws.tmeta[tile] = make_uint2(0u, kInvalid);
-            else tile_chain(parents, ws, N, tile);
-        }
-        return;
-    }
-    const uint32_t base = (blockIdx.x - cblocks) * kCandBlock;
+// Arrival word of a node completed by the tile climb: both tags are phase 0
+// (tag = owner + 1 < 2^23), so their sum stays below 2^24; later phases add
+// a tag of at least 2^24, a deferred node carries bit 63 (kDeferred) and a
+// hole's word is all ones.
+__device__ __forceinline__ bool tile_complete(uint64_t v) { return complete(v) && (uint32_t)(v >> 32) < (1u << 24); }
+
+// One workgroup's nodes (node[t] for lane t, kInvalid for none).
+__device__ __forceinline__ void cand_block(const kr_vec3* __restrict__ V, TreeWs& ws, uint32_t I, CandTile& T,
+                                           uint32_t node, bool bad) {
+    const uint32_t t = threadIdx.x;
     if (t < 16) T.hist[t] = 0;
-    const bool bad = *ws.error != 0;
+    T.node[t] = node;
     __syncthreads();
     // 1. the lane's own node: distinct ids and the farthest slab pair
     {
-        const uint32_t node = base + t;
         uint32_t cnt = 0;
         if (node < I) {
-            const uint64_t vw = ws.visit[node];
-            if (hole(vw)) {
-                Cand c = {};
-                c.degen = 1;
-                ws.cand[node] = c;
-            } else if (bad || !complete(vw)) {
-                // malformed tree: never read the ids of a node that did not complete
-                if (!complete(vw)) atomicOr(ws.error, 8u);
+            if (hole(ws.visit[node]) || bad) {
                 Cand c = {};
                 c.degen = 1;
                 ws.cand[node] = c;
@@ -1226,12 +1249,51 @@ __global__ __launch_bounds__(kCandBlock, 4) void candidates(const kr_vec3* __res
     const uint32_t s = T.order[t];
     const int cnt = T.cnt[s];
     const int n = wave_max_i32(cnt);  // every lane is active here
-    if (cnt == 0) return;
-    const uint32_t node = base + s;
-    if (n <= 4) cand_distinct<4>(V, ws, T, s, node, cnt);
-    else if (n <= 6) cand_distinct<6>(V, ws, T, s, node, cnt);
-    else if (n <= 8) cand_distinct<8>(V, ws, T, s, node, cnt);
-    else cand_distinct<14>(V, ws, T, s, node, cnt);
+    if (cnt != 0) {
+        const uint32_t nd = T.node[s];
+        if (n <= 4) cand_distinct<4>(V, ws, T, s, nd, cnt);
+        else if (n <= 6) cand_distinct<6>(V, ws, T, s, nd, cnt);
+        else if (n <= 8) cand_distinct<8>(V, ws, T, s, nd, cnt);
+        else cand_distinct<14>(V, ws, T, s, nd, cnt);
+    }
+    __syncthreads();  // T is reused by the workgroup's next round
+}
+
+// The candidate search, in two launches.  kUpper = false: the nodes the
+// tile climb completed (index order, one round per workgroup), launched
+// on the side stream beside the top-of-tree climb phases, which leave most
+// of the chip idle; holes get an AABB record.  kUpper = true: the nodes the
+// later phases completed (their list), and workgroups below `cblocks` find
+// the refit tiles' ancestor chains.
+template <bool kUpper>
+__global__ __launch_bounds__(kCandBlock, 4) void candidates(const kr_vec3* __restrict__ V,
+                                                            const uint32_t* __restrict__ parents, TreeWs ws,
+                                                            uint32_t N, uint32_t cblocks) {
+    __shared__ CandTile T;
+    const uint32_t t = threadIdx.x, I = N - 1;
+    if (kUpper && blockIdx.x < cblocks) {
+        const uint32_t tile = blockIdx.x * kCandBlock + t;
+        if (tile < (N + kRTile - 1) / kRTile) {
+            if (*ws.error != 0) ws.tmeta[tile] = make_uint2(0u, kInvalid);
+            else tile_chain(parents, ws, N, tile);
+        }
+        return;
+    }
+    const bool bad = *ws.error != 0;
+    if (!kUpper) {
+        const uint32_t node = blockIdx.x * kCandBlock + t;
+        bool mine = false;
+        if (node < I) {
+            const uint64_t vw = ws.visit[node];
+            mine = hole(vw) || tile_complete(vw);
+        }
+        cand_block(V, ws, I, T, mine ? node : kInvalid, bad);
+        return;
+    }
+    const uint32_t nb = gridDim.x - cblocks, nup = min(ws.nlist[kUpperList], I);
+    const uint32_t* up = ws.list + (size_t)kUpperList * (I + 1);
+    for (uint32_t base = (blockIdx.x - cblocks) * kCandBlock; base < nup; base += nb * kCandBlock)  // uniform
+        cand_block(V, ws, I, T, base + t < nup ? up[base + t] : kInvalid, bad);
 }
 
 // The candidate search in reference mode on the nodes the distinct-mode
@@ -1647,6 +1709,7 @@ __global__ __launch_bounds__(kStage) void finalize(TreeWs ws, kr_obb3* obbs, kr_
     float* mt = &st.mt[threadIdx.x * 17];
     if (node < I) {
         const uint64_t vw = ws.visit[node];
+        if (!complete(vw)) atomicOr(ws.error, 8u);  // a malformed tree: the node never completed
         if (*ws.error != 0 || !complete(vw) || hole(vw)) {
             stage_nan(st, threadIdx.x);
         } else {
@@ -1737,7 +1800,7 @@ size_t tree_carve(TreeWs* ws, void* base, size_t cap, uint32_t N) {
 namespace {
 struct SideStream {
     hipStream_t s = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr, cand = nullptr;
 };
 constexpr int kMaxDevices = 64;
 std::mutex g_side_mu;
@@ -1752,6 +1815,7 @@ kr_error side_stream(SideStream** out) {
         OBVH_HIP_TRY(hipStreamCreateWithFlags(&ss.s, hipStreamNonBlocking));
         OBVH_HIP_TRY(hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming));
         OBVH_HIP_TRY(hipEventCreateWithFlags(&ss.join, hipEventDisableTiming));
+        OBVH_HIP_TRY(hipEventCreateWithFlags(&ss.cand, hipEventDisableTiming));
     }
     *out = &ss;
     return nullptr;
@@ -1774,22 +1838,24 @@ kr_error tree_launch(const kr_bvh_node* nodes, const uint32_t* parents, const kr
     hipLaunchKernelGGL(tree_clear, dim3(1024), dim3(256), 0, stream, ws.visit, I + 1, ws);
     if (bm.K > 1)
         hipLaunchKernelGGL(batch_holes, dim3(min((bm.K + 255) / 256, 1024u)), dim3(256), 0, stream, ws, bm);
-    // climb: 512-leaf tiles, then 16384-leaf workgroups, then one workgroup
+    // climb: 512-leaf tiles, then 32768-leaf workgroups, then one workgroup
     // for the rest
     hipLaunchKernelGGL(climb_leaves, blocks(N, kTile), dim3(kTile), 0, stream, nodes, parents, V, F, ws, N);
-    // The leaf boxes need none of the other kernels: they run on the side
-    // stream beside the 16384-leaf climb phase (60-odd workgroups), which
-    // they slow by about half their own time.  Beside the last phase (one
-    // workgroup) or finalize they saved nothing: the fork's event adds a gap
-    // of ~7 us to the launch stream and the co-running kernel stretches.
-    auto fork_leaf_boxes = [&]() -> kr_error {
-        OBVH_HIP_TRY(hipEventRecord(side->fork, stream));
-        OBVH_HIP_TRY(hipStreamWaitEvent(side->s, side->fork, 0));
-        hipLaunchKernelGGL(leaf_boxes, blocks(N, kStage), dim3(kStage), 0, side->s, nodes, V, F, obbs, mats, N, bm);
-        OBVH_HIP_TRY(hipEventRecord(side->join, side->s));
-        return nullptr;
-    };
-    if (kr_error e = fork_leaf_boxes()) return e;
+    // The side stream takes what needs only the tile climb: the candidate
+    // search of the nodes it completed (nearly all of them; VALU-bound),
+    // then the leaf boxes (HBM-bound), beside the top-of-tree climb phases,
+    // which are latency-bound on a few dozen workgroups.  The launch stream
+    // waits for the search before the refit and for the leaf boxes at the
+    // end of the call.
+    OBVH_HIP_TRY(hipEventRecord(side->fork, stream));
+    OBVH_HIP_TRY(hipStreamWaitEvent(side->s, side->fork, 0));
+    if (I > 0) {
+        hipLaunchKernelGGL(candidates<false>, blocks(I, kCandBlock), dim3(kCandBlock), 0, side->s, V, parents, ws, N,
+                           0u);
+        OBVH_HIP_TRY(hipEventRecord(side->cand, side->s));
+    }
+    hipLaunchKernelGGL(leaf_boxes, blocks(N, kStage), dim3(kStage), 0, side->s, nodes, V, F, obbs, mats, N, bm);
+    OBVH_HIP_TRY(hipEventRecord(side->join, side->s));
     if (N > kTile) {
         uint32_t phase = 1;
         for (uint32_t chunk : {kChunk1, 0u}) {
@@ -1801,9 +1867,11 @@ kr_error tree_launch(const kr_bvh_node* nodes, const uint32_t* parents, const kr
         }
     }
     if (I > 0) {
+        // the nodes the phases completed, and the refit tiles' chains
         const uint32_t cb = (((N + kRTile - 1) / kRTile) + kCandBlock - 1) / kCandBlock;
-        hipLaunchKernelGGL(candidates, dim3(cb + (I + kCandBlock - 1) / kCandBlock), dim3(kCandBlock), 0, stream, V,
-                           parents, ws, N, cb);
+        const uint32_t ub = min((I + kCandBlock - 1) / kCandBlock, 1024u);
+        hipLaunchKernelGGL(candidates<true>, dim3(cb + ub), dim3(kCandBlock), 0, stream, V, parents, ws, N, cb);
+        OBVH_HIP_TRY(hipStreamWaitEvent(stream, side->cand, 0));
         hipLaunchKernelGGL(cand_redo, dim3(kRedoBlocks), dim3(1024), 0, stream, V, ws, I);
         hipLaunchKernelGGL(refit, blocks(N, kRTile), dim3(kRTile), 0, stream, nodes, parents, V, F, ws, N);
         hipLaunchKernelGGL(finalize, blocks(I, kStage), dim3(kStage), 0, stream, ws, obbs, mats, I, bm);
