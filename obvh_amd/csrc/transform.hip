@@ -301,6 +301,7 @@ __global__ void tree_clear(uint64_t* visit, uint32_t n, TreeWs ws) {
     if (t == 0) {
         *ws.error = 0;
         for (int k = 0; k < kMaxPhases; ++k) ws.nlist[k] = 0;
+        *ws.phase_done = 0;
     }
     for (uint32_t i = t; i < n; i += gridDim.x * blockDim.x) visit[i] = 0;
 }
@@ -658,13 +659,13 @@ __device__ __forceinline__ uint32_t phase_slot(PhaseTile& P, uint32_t p) {
     return kInvalid;
 }
 
-__global__ __launch_bounds__(1024) void climb_phase(const kr_bvh_node* __restrict__ nodes,
-                                                    const uint32_t* __restrict__ parents, TreeWs ws, uint32_t N,
-                                                    uint32_t phase, uint32_t chunk) {
-    __shared__ PhaseTile P;
+// One phase as workgroup `wg` of `nwg`.
+__device__ __forceinline__ void phase_pass(PhaseTile& P, const kr_bvh_node* __restrict__ nodes,
+                                           const uint32_t* __restrict__ parents, TreeWs& ws, uint32_t N,
+                                           uint32_t phase, uint32_t chunk, uint32_t wg, uint32_t nwg) {
     const uint32_t I = N - 1, total = 2 * N - 1, t = threadIdx.x;
-    const uint32_t tag = make_tag(phase, blockIdx.x);
-    const uint64_t lo = (uint64_t)blockIdx.x * chunk, hi = lo + chunk;  // owned node indices
+    const uint32_t tag = make_tag(phase, wg);
+    const uint64_t lo = (uint64_t)wg * chunk, hi = lo + chunk;  // owned node indices
     const uint32_t nin = min(ws.nlist[phase - 1], I);
     const uint32_t* in = ws.list + (size_t)(phase - 1) * (I + 1);
     for (uint32_t h = t; h < (uint32_t)kPSlots; h += blockDim.x) {
@@ -675,7 +676,7 @@ __global__ __launch_bounds__(1024) void climb_phase(const kr_bvh_node* __restric
         P.nq = 0;
         P.ndone = 0;
     }
-    const bool last = gridDim.x == 1;
+    const bool last = nwg == 1;
     auto completed = [&](uint32_t node) {
         const uint32_t d = atomicAdd(&P.ndone, 1u);
         if (d < (uint32_t)kPQueue) {
@@ -813,6 +814,34 @@ __global__ __launch_bounds__(1024) void climb_phase(const kr_bvh_node* __restric
     const uint32_t nd = min(P.ndone, (uint32_t)kPQueue);
     for (uint32_t i = t; i < nd; i += blockDim.x)
         if (P.dbase + i < I) ws.list[(size_t)kUpperList * (I + 1) + P.dbase + i] = P.done[i];
+}
+
+// Phase 1 on `gridDim.x` workgroups; the last workgroup to finish (one
+// agent-scope release per workgroup, an acquire in the last) then runs the
+// last phase over the whole tree on the CU it holds, so that the last phase
+// does not wait for a CU to drain while the side stream fills the chip.  A
+// launch with one workgroup runs its phase as the last.
+__global__ __launch_bounds__(1024) void climb_phase(const kr_bvh_node* __restrict__ nodes,
+                                                    const uint32_t* __restrict__ parents, TreeWs ws, uint32_t N,
+                                                    uint32_t phase, uint32_t chunk) {
+    __shared__ PhaseTile P;
+    __shared__ uint32_t finished_last;
+    uint32_t wg = blockIdx.x, nwg = gridDim.x;
+    for (;;) {
+        phase_pass(P, nodes, parents, ws, N, phase, chunk, wg, nwg);
+        if (nwg == 1) return;
+        __syncthreads();
+        if (threadIdx.x == 0)
+            finished_last =
+                __hip_atomic_fetch_add(ws.phase_done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == nwg - 1;
+        __syncthreads();
+        if (!finished_last) return;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        ++phase;
+        chunk = N;
+        wg = 0;
+        nwg = 1;
+    }
 }
 
 // ------------------------------------------------------- candidates --
@@ -1781,6 +1810,7 @@ size_t tree_carve(TreeWs* ws, void* base, size_t cap, uint32_t N) {
     ws->visit = c.take<uint64_t>(I + 1);
     ws->error = c.take<uint32_t>(8);
     ws->nlist = ws->error ? ws->error + 2 : nullptr;  // error[2 .. 2 + kMaxPhases)
+    ws->phase_done = ws->error ? ws->error + 6 : nullptr;
     ws->list = c.take<uint32_t>((size_t)kMaxPhases * (I + 1));
     ws->span = c.take<uint2>(total);
     ws->count = c.take<uint32_t>(I + 1);
@@ -1856,15 +1886,9 @@ kr_error tree_launch(const kr_bvh_node* nodes, const uint32_t* parents, const kr
     }
     hipLaunchKernelGGL(leaf_boxes, blocks(N, kStage), dim3(kStage), 0, side->s, nodes, V, F, obbs, mats, N, bm);
     OBVH_HIP_TRY(hipEventRecord(side->join, side->s));
-    if (N > kTile) {
-        uint32_t phase = 1;
-        for (uint32_t chunk : {kChunk1, 0u}) {
-            const bool last = (chunk == 0) || chunk >= N;
-            const uint32_t c = last ? N : chunk;
-            hipLaunchKernelGGL(climb_phase, blocks(N, c), dim3(1024), 0, stream, nodes, parents, ws, N, phase, c);
-            ++phase;
-            if (last) break;
-        }
+    if (N > kTile) {  // phase 1 (and its last workgroup phase 2), or one phase for a small tree
+        const uint32_t c = N > kChunk1 ? kChunk1 : N;
+        hipLaunchKernelGGL(climb_phase, blocks(N, c), dim3(1024), 0, stream, nodes, parents, ws, N, 1u, c);
     }
     if (I > 0) {
         // the nodes the phases completed, and the refit tiles' chains

@@ -18,7 +18,8 @@ constexpr int kMaxPhases = 4;
 
 struct TreeWs {
     uint64_t* visit;   // per internal node: arrivals (low word) + sum of arrival tags (high word)
-    uint32_t* nlist;   // per climb phase: number of deferred nodes
+    uint32_t* nlist;   // per node list: number of entries
+    uint32_t* phase_done;  // workgroups of climb phase 1 that finished
     uint32_t* list;    // per climb phase: nodes deferred to the next phase
     uint2* span;       // per node: first and last leaf position below it (bit 31 of .y: the
                        // leaves below are not every position from first to last)
