@@ -3,34 +3,37 @@
 // Method of the reference GPU transform (src/ads/common/cuda/transforms.cu
 // :213-964), restructured for CDNA4:
 //
-//  leaf_boxes    the leaf OBB and its matrix (transforms.cu:803-834), on a
-//                second stream beside the top-of-tree climb.
 //  climb_leaves  one lane per leaf, 512-leaf workgroups: the leaf's 7-slab
 //                k-DOP (3 vertices, first vertex wins ties), then a bottom-up climb
 //                in which the second child to arrive at a parent merges the
 //                two k-DOPs.  Arrivals at parents inside the workgroup's range
 //                are counted in LDS; the rest use a 64-bit global arrival
 //                word tagged with the workgroup, and parents whose children
-//                finished in different workgroups are deferred.
-//  climb_phase   the deferred parents, in 16384-leaf workgroups and then one
-//                workgroup for the rest (workgroup-scope ordering only; the
-//                kernel boundary publishes everything else).  The merge takes
-//                the right child's extreme only if strictly better, so ties go
-//                to the left child and the selected vertex ids do not depend
-//                on scheduling.  k-DOPs carry vertex ids, not positions
-//                (7 x {min, id, max, id} = 112 B per node).
-//  list_wide     the internal nodes whose span holds a whole aligned refit
-//                tile, for the refit's ancestor lookup.
+//                finished in different workgroups are deferred.  Then the
+//                leaf OBBs and matrices (transforms.cu:803-834).
+//  climb_phase   the deferred parents, in 32768-leaf workgroups whose last
+//                one to finish then climbs the rest of the tree; arrivals
+//                within a workgroup are counted, and k-DOPs handed over, in
+//                LDS.  The merge takes the right child's extreme only if
+//                strictly better, so ties go to the left child and the
+//                selected vertex ids do not depend on scheduling.  k-DOPs
+//                carry vertex ids, not positions (7 x {min, id, max, id} =
+//                112 B per node); spans carry an exact gap flag from leaf
+//                counts.
 //  candidates    one lane per internal node: gather the 14 extremal vertices
 //                and run the base-triangle + ditetrahedron search in
-//                registers (transforms.cu:413-756).
+//                registers (transforms.cu:413-756); the nodes the tile climb
+//                completed on a second stream beside climb_phase, the rest
+//                after it, together with each refit tile's ancestor chain.
+//  cand_redo     the search in reference mode for nodes with non-finite or
+//                huge extremal points.
 //  refit         exact projection range of every subtree vertex on each
 //                node's candidate axes (transforms.cu:836-900): per-lane walks
 //                below each wave's common ancestor, DPP wave reductions above
 //                it (both reading the tile's own nodes from an LDS copy), a
 //                tile-wide pass over the tile's distinct vertices for the
-//                ancestors of the tile (found in the wide-node list), LDS
-//                keys per workgroup, filtered global atomics.
+//                tile's ancestor chain, LDS keys per workgroup, global
+//                atomics.
 //  finalize      one lane per internal node: OBB vs AABB by half area, extent
 //                floor, inverse(T * R * S) (transforms.cu:905-964).
 //
@@ -188,17 +191,20 @@ __device__ __forceinline__ uint32_t batch_out_index(const BatchMap& m, uint32_t 
     return 2 * Lt - t + (c - Lt);
 }
 
-// Output staging for 256 consecutive nodes: boxes at row stride 15 floats,
-// matrices at row stride 17 (odd strides: no LDS bank conflicts), written
-// back to HBM as contiguous dwords.
+// Output staging for R consecutive nodes (R = the workgroup size): boxes at
+// row stride 15 floats, matrices at row stride 17 (odd strides: no LDS bank
+// conflicts), written back to HBM as contiguous dwords.
 constexpr int kStage = 256;
-struct Stage {
-    float ob[kStage * 15];
-    float mt[kStage * 17];
-    uint32_t orow[kStage];  // output node of each row (batches)
+template <int R>
+struct StageT {
+    float ob[R * 15];
+    float mt[R * 17];
+    uint32_t orow[R];  // output node of each row (batches)
 };
+using Stage = StageT<kStage>;
 
-__device__ __forceinline__ void stage_nan(Stage& st, uint32_t r) {
+template <int R>
+__device__ __forceinline__ void stage_nan(StageT<R>& st, uint32_t r) {
     const float nan = __builtin_nanf("");
 #pragma unroll
     for (int q = 0; q < 15; ++q) st.ob[r * 15 + q] = nan;
@@ -209,24 +215,25 @@ __device__ __forceinline__ void stage_nan(Stage& st, uint32_t r) {
 // Rows first .. first + cnt - 1 (combined indices) to their output nodes: the
 // same node indices, or for a batch each tree's own layout (runs of
 // consecutive rows stay consecutive; top nodes are dropped).
-__device__ __forceinline__ void stage_flush(Stage& st, kr_obb3* obbs, kr_mat4* mats, size_t first, uint32_t cnt,
+template <int R>
+__device__ __forceinline__ void stage_flush(StageT<R>& st, kr_obb3* obbs, kr_mat4* mats, size_t first, uint32_t cnt,
                                             const BatchMap& m) {
     if (m.K == 0) {
         float* go = reinterpret_cast<float*>(obbs) + first * 15;
-        for (uint32_t j = threadIdx.x; j < cnt * 15; j += kStage) go[j] = st.ob[j];
+        for (uint32_t j = threadIdx.x; j < cnt * 15; j += R) go[j] = st.ob[j];
         float* gm = reinterpret_cast<float*>(mats) + first * 16;
-        for (uint32_t j = threadIdx.x; j < cnt * 16; j += kStage) gm[j] = st.mt[(j >> 4) * 17 + (j & 15)];
+        for (uint32_t j = threadIdx.x; j < cnt * 16; j += R) gm[j] = st.mt[(j >> 4) * 17 + (j & 15)];
         return;
     }
     if (threadIdx.x < cnt) st.orow[threadIdx.x] = batch_out_index(m, (uint32_t)first + threadIdx.x);
     __syncthreads();
     float* go = reinterpret_cast<float*>(obbs);
-    for (uint32_t j = threadIdx.x; j < cnt * 15; j += kStage) {
+    for (uint32_t j = threadIdx.x; j < cnt * 15; j += R) {
         const uint32_t r = j / 15, o = st.orow[r];
         if (o != kInvalid) go[(size_t)o * 15 + (j - r * 15)] = st.ob[j];
     }
     float* gm = reinterpret_cast<float*>(mats);
-    for (uint32_t j = threadIdx.x; j < cnt * 16; j += kStage) {
+    for (uint32_t j = threadIdx.x; j < cnt * 16; j += R) {
         const uint32_t o = st.orow[j >> 4];
         if (o != kInvalid) gm[(size_t)o * 16 + (j & 15)] = st.mt[(j >> 4) * 17 + (j & 15)];
     }
@@ -401,29 +408,6 @@ __device__ __forceinline__ bool leaf_kdop_global(const kr_bvh_node* nodes, const
     return true;
 }
 
-// Leaf boxes (transforms.cu:803-834), 256 leaves per workgroup, output
-// staged in LDS and written back contiguously.
-__global__ __launch_bounds__(kStage) void leaf_boxes(const kr_bvh_node* __restrict__ nodes,
-                                                     const kr_vec3* __restrict__ V,
-                                                     const kr_uvec4* __restrict__ F, kr_obb3* obbs,
-                                                     kr_mat4* mats, uint32_t N, BatchMap bm) {
-    __shared__ Stage st;
-    const uint32_t I = N - 1;
-    const uint32_t base = blockIdx.x * kStage, pos = base + threadIdx.x;
-    if (pos < N) {
-        const uint32_t f = nodes[I + pos].u.primitivesOffset;
-        if (f < N) {
-            const kr_uvec4 face = F[f];
-            leaf_obb(ldv(V, face.x), ldv(V, face.y), ldv(V, face.z), &st.ob[threadIdx.x * 15],
-                     &st.mt[threadIdx.x * 17]);
-        } else {
-            stage_nan(st, threadIdx.x);  // the climb reports the bad face id
-        }
-    }
-    __syncthreads();
-    stage_flush(st, obbs, mats, (size_t)I + base, min((uint32_t)kStage, N - base), bm);
-}
-
 // Phase 0 of the bottom-up k-DOP pass.  Workgroup w owns leaf positions
 // [w*kTile, (w+1)*kTile) and the internal nodes with the same indices (for a
 // Karras-ordered tree these are the internal nodes whose leaves lie in the
@@ -439,7 +423,9 @@ __global__ __launch_bounds__(kStage) void leaf_boxes(const kr_bvh_node* __restri
 // children finished in different workgroups.  Finally tile nodes reached by
 // one child only are folded into the global arrival words, and the k-DOPs,
 // spans and arrival words of tile nodes completed here are written back
-// contiguously.  The merge keeps the left child's extreme on ties.
+// contiguously.  The merge keeps the left child's extreme on ties.  Last,
+// the tile's leaf boxes (transforms.cu:803-834) from the triangles each lane
+// loaded, staged in the same LDS and written back contiguously.
 constexpr uint32_t kTile = kChunk0;
 
 struct ClimbTile {
@@ -452,6 +438,10 @@ struct ClimbTile {
     uint32_t par_int[kTile];    // parent of internal node lo + s
     uint32_t par_leaf[kTile];   // parent of the leaf at position lo + s
 };
+union ClimbLds {
+    ClimbTile L;
+    StageT<kTile> st;           // after the climb: the leaf boxes
+};
 
 __device__ __forceinline__ void tile_leaf_kdop(const ClimbTile& L, const kr_vec3* V, uint32_t s, Kdop& k) {
     const uint32_t vid[3] = {L.lid[s][0], L.lid[s][1], L.lid[s][2]};
@@ -461,8 +451,10 @@ __device__ __forceinline__ void tile_leaf_kdop(const ClimbTile& L, const kr_vec3
 __global__ __launch_bounds__(kTile) void climb_leaves(const kr_bvh_node* __restrict__ nodes,
                                                       const uint32_t* __restrict__ parents,
                                                       const kr_vec3* __restrict__ V,
-                                                      const kr_uvec4* __restrict__ F, TreeWs ws, uint32_t N) {
-    __shared__ ClimbTile L;
+                                                      const kr_uvec4* __restrict__ F, TreeWs ws, uint32_t N,
+                                                      kr_obb3* obbs, kr_mat4* mats, BatchMap bm) {
+    __shared__ ClimbLds U;
+    ClimbTile& L = U.L;
     const uint32_t I = N - 1, total = 2 * N - 1;
     const uint32_t t = threadIdx.x;
     const uint32_t lo = blockIdx.x * kTile;
@@ -472,6 +464,7 @@ __global__ __launch_bounds__(kTile) void climb_leaves(const kr_bvh_node* __restr
     // tile topology and the lane's leaf
     Kdop k;
     bool climbing = false;
+    f3 v0 = f3{0, 0, 0}, v1 = v0, v2 = v0;  // the lane's triangle, for its leaf box
     if (pos < N) {
         const uint32_t leaf = I + pos;
         L.par_leaf[t] = parents[leaf];
@@ -482,7 +475,10 @@ __global__ __launch_bounds__(kTile) void climb_leaves(const kr_bvh_node* __restr
             L.lid[t][0] = vid[0];
             L.lid[t][1] = vid[1];
             L.lid[t][2] = vid[2];
-            leaf_kdop(ldv(V, vid[0]), ldv(V, vid[1]), ldv(V, vid[2]), vid, k);
+            v0 = ldv(V, vid[0]);
+            v1 = ldv(V, vid[1]);
+            v2 = ldv(V, vid[2]);
+            leaf_kdop(v0, v1, v2, vid, k);
             climbing = true;
         } else {
             L.lid[t][0] = kInvalid;  // never climbs, never read as a sibling
@@ -610,6 +606,14 @@ __global__ __launch_bounds__(kTile) void climb_leaves(const kr_bvh_node* __restr
         span_store(ws, pos, L.span[t], L.cnt[t]);
         ws.visit[pos] = ((uint64_t)tag << 33) | 2ull;
     }
+    // the leaf boxes (a bad face id gives a NaN row; the climb reported it)
+    __syncthreads();
+    if (pos < N) {
+        if (climbing) leaf_obb(v0, v1, v2, &U.st.ob[t * 15], &U.st.mt[t * 17]);
+        else stage_nan(U.st, t);
+    }
+    __syncthreads();
+    stage_flush(U.st, obbs, mats, (size_t)I + lo, min(kTile, N - lo), bm);
 }
 
 // Phase >= 1 of the bottom-up k-DOP pass.  Workgroup w takes the nodes the
@@ -1824,13 +1828,13 @@ size_t tree_carve(TreeWs* ws, void* base, size_t cap, uint32_t N) {
 }
 
 // Per device: a second stream (default priority) and the events that fork
-// leaf_boxes onto it and join it back.  One lock covers a whole call's
+// the first candidate launch onto it and join it back.  One lock covers a whole call's
 // enqueue sequence, so calls from several host threads cannot interleave
 // their record / wait pairs.
 namespace {
 struct SideStream {
     hipStream_t s = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr, cand = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
 };
 constexpr int kMaxDevices = 64;
 std::mutex g_side_mu;
@@ -1845,7 +1849,6 @@ kr_error side_stream(SideStream** out) {
         OBVH_HIP_TRY(hipStreamCreateWithFlags(&ss.s, hipStreamNonBlocking));
         OBVH_HIP_TRY(hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming));
         OBVH_HIP_TRY(hipEventCreateWithFlags(&ss.join, hipEventDisableTiming));
-        OBVH_HIP_TRY(hipEventCreateWithFlags(&ss.cand, hipEventDisableTiming));
     }
     *out = &ss;
     return nullptr;
@@ -1870,22 +1873,19 @@ kr_error tree_launch(const kr_bvh_node* nodes, const uint32_t* parents, const kr
         hipLaunchKernelGGL(batch_holes, dim3(min((bm.K + 255) / 256, 1024u)), dim3(256), 0, stream, ws, bm);
     // climb: 512-leaf tiles, then 32768-leaf workgroups, then one workgroup
     // for the rest
-    hipLaunchKernelGGL(climb_leaves, blocks(N, kTile), dim3(kTile), 0, stream, nodes, parents, V, F, ws, N);
-    // The side stream takes what needs only the tile climb: the candidate
-    // search of the nodes it completed (nearly all of them; VALU-bound),
-    // then the leaf boxes (HBM-bound), beside the top-of-tree climb phases,
-    // which are latency-bound on a few dozen workgroups.  The launch stream
-    // waits for the search before the refit and for the leaf boxes at the
-    // end of the call.
-    OBVH_HIP_TRY(hipEventRecord(side->fork, stream));
-    OBVH_HIP_TRY(hipStreamWaitEvent(side->s, side->fork, 0));
+    hipLaunchKernelGGL(climb_leaves, blocks(N, kTile), dim3(kTile), 0, stream, nodes, parents, V, F, ws, N, obbs, mats,
+                       bm);
+    // The side stream takes the candidate search of the nodes the tile climb
+    // completed (nearly all of them; VALU-bound) beside the top-of-tree climb
+    // phases, which are latency-bound on a few dozen workgroups.  The launch
+    // stream joins it before the refit.
     if (I > 0) {
+        OBVH_HIP_TRY(hipEventRecord(side->fork, stream));
+        OBVH_HIP_TRY(hipStreamWaitEvent(side->s, side->fork, 0));
         hipLaunchKernelGGL(candidates<false>, blocks(I, kCandBlock), dim3(kCandBlock), 0, side->s, V, parents, ws, N,
                            0u);
-        OBVH_HIP_TRY(hipEventRecord(side->cand, side->s));
+        OBVH_HIP_TRY(hipEventRecord(side->join, side->s));
     }
-    hipLaunchKernelGGL(leaf_boxes, blocks(N, kStage), dim3(kStage), 0, side->s, nodes, V, F, obbs, mats, N, bm);
-    OBVH_HIP_TRY(hipEventRecord(side->join, side->s));
     if (N > kTile) {  // phase 1 (and its last workgroup phase 2), or one phase for a small tree
         const uint32_t c = N > kChunk1 ? kChunk1 : N;
         hipLaunchKernelGGL(climb_phase, blocks(N, c), dim3(1024), 0, stream, nodes, parents, ws, N, 1u, c);
@@ -1895,14 +1895,13 @@ kr_error tree_launch(const kr_bvh_node* nodes, const uint32_t* parents, const kr
         const uint32_t cb = (((N + kRTile - 1) / kRTile) + kCandBlock - 1) / kCandBlock;
         const uint32_t ub = min((I + kCandBlock - 1) / kCandBlock, 1024u);
         hipLaunchKernelGGL(candidates<true>, dim3(cb + ub), dim3(kCandBlock), 0, stream, V, parents, ws, N, cb);
-        OBVH_HIP_TRY(hipStreamWaitEvent(stream, side->cand, 0));
+        OBVH_HIP_TRY(hipStreamWaitEvent(stream, side->join, 0));
         hipLaunchKernelGGL(cand_redo, dim3(kRedoBlocks), dim3(1024), 0, stream, V, ws, I);
         hipLaunchKernelGGL(refit, blocks(N, kRTile), dim3(kRTile), 0, stream, nodes, parents, V, F, ws, N);
         hipLaunchKernelGGL(finalize, blocks(I, kStage), dim3(kStage), 0, stream, ws, obbs, mats, I, bm);
     }
     if (ids)
         hipLaunchKernelGGL(copy_ids, blocks(2 * N - 1, 256), dim3(256), 0, stream, nodes, V, F, ws, ids, 2 * N - 1);
-    OBVH_HIP_TRY(hipStreamWaitEvent(stream, side->join, 0));
     OBVH_HIP_TRY(hipGetLastError());
     return nullptr;
 }
@@ -2033,8 +2032,8 @@ kr_error tree_batch_launch(const kr_bvh_node* nodes, const uint32_t* parents, co
                        w.parents, w.berr, total_in);
     hipLaunchKernelGGL(batch_gather_faces, grid(Nsum), dim3(256), 0, stream, m, vert_off, F, w.faces, w.berr);
     OBVH_HIP_TRY(hipGetLastError());
-    // leaf_boxes and finalize write every node's box and matrix straight to
-    // its place in the caller's per-tree layout
+    // climb_leaves and finalize write every node's box and matrix straight
+    // to its place in the caller's per-tree layout
     return tree_launch(w.nodes, w.parents, V, w.faces, obbs, mats, nullptr, Nsum, w.tree_ws, w.tree_ws_bytes, stream,
                        &m);
 }
