@@ -55,6 +55,12 @@ def pmc_traffic(profile: str):
     return None
 
 
+# The kernels of one transform call; the roofline's device time is the call's,
+# on the launch stream.
+CALL_KERNELS = ("obb_tree call: tree_clear, climb_leaves, climb_phase, candidates x 2 (the first on a second "
+                "stream beside climb_phase), cand_redo, refit, finalize")
+
+
 def pmc_value(profile: str, key: str):
     """A `key: value` line of a committed profile summary, or None."""
     path = os.path.join(ROOT, "profiles", profile)
@@ -712,14 +718,14 @@ def main() -> int:
         roofline_c3["traffic"] = pmc_traffic("transform_hairball2m_pmc.txt")
         roofline_c3["workload"] = (f"ATRBVH (treelet 7, 2 passes) -> OBB-tree transform, hairball of 50000 tubes: "
                                    f"{c3['triangles']} triangles, {c3['internal_nodes']} internal nodes")
-        roofline_c3["kernel"] = "obb_tree call (7 kernels), device time on the launch stream"
+        roofline_c3["kernel"] = CALL_KERNELS
     valu = None
     vi, vus = (pmc_value("transform_icosphere224_valu.txt", "candidates_valu_wave_instructions_per_call"),
                pmc_value("transform_icosphere224_valu.txt", "candidates_avg_us"))
     if vi and vus:
         # 1024 SIMDs x 2.4 GHz / 4 cycles per wave64 VALU instruction
         peak = 1024 * 2.4e9 / 4
-        valu = {"bound": "valu_issue", "kernel": "candidates", "achieved": round(vi / (vus * 1e-6) / 1e9, 1),
+        valu = {"bound": "valu_issue", "kernel": "candidates (the launch over the tile-completed nodes)", "achieved": round(vi / (vus * 1e-6) / 1e9, 1),
                 "peak": round(peak / 1e9, 1), "unit": "G wave-instructions/s",
                 "frac": round(vi / (vus * 1e-6) / peak, 4), "source": "profiles/transform_icosphere224_valu.txt"}
 
@@ -751,7 +757,7 @@ def main() -> int:
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": pmc_traffic("transform_icosphere224_pmc.txt") if args.freq == 224 else None,
-            "kernel": "obb_tree call (leaf_boxes, climb_leaves, climb_phase x2, candidates, refit, finalize)",
+            "kernel": CALL_KERNELS,
             "bytes_per_call": balg,
             "device_ms": round(dev_ms_mean, 5),
         },
