@@ -435,18 +435,32 @@ __device__ __forceinline__ void set_cand(P pts, uint64_t sbeg, uint64_t send, ui
     // the order of the reduction does not matter.
     float v[14];
     uint32_t ix[14];
+    const f3 pt0 = pts.load(sbeg);  // for the NaN rule below, read beside the partials
     {
         const int side = lane & 15;
         const bool mn = side < 7;
         float a = mn ? FLT_MAX * 2.0f : -FLT_MAX * 2.0f;
         uint32_t ai = kNoIdx;
         if (side < 14) {
-            for (uint32_t j = (uint32_t)(lane >> 4); j < nt; j += 4) {
-                const float ov = part1_v[(size_t)(ft + j) * 14 + side];
-                const uint32_t oi = part1_i[(size_t)(ft + j) * 14 + side];
-                const bool better =
-                    (ai == kNoIdx) || (mn ? pair_better<true>(a, ai, ov, oi) : pair_better<false>(a, ai, ov, oi));
-                if (better) { a = ov; ai = oi; }
+            // eight tiles' pairs are read before any is compared: one memory
+            // round trip per 32 tiles instead of one per 4
+            constexpr int kB = 8;
+            for (uint32_t j0 = (uint32_t)(lane >> 4); j0 < nt; j0 += 4 * kB) {
+                float ov[kB];
+                uint32_t oi[kB];
+#pragma unroll
+                for (int u = 0; u < kB; ++u) {
+                    const uint32_t j = j0 + 4u * u;
+                    ov[u] = j < nt ? part1_v[(size_t)(ft + j) * 14 + side] : 0.0f;
+                    oi[u] = j < nt ? part1_i[(size_t)(ft + j) * 14 + side] : kNoIdx;
+                }
+#pragma unroll
+                for (int u = 0; u < kB; ++u) {
+                    if (j0 + 4u * u >= nt) continue;
+                    const bool better = (ai == kNoIdx) || (mn ? pair_better<true>(a, ai, ov[u], oi[u])
+                                                              : pair_better<false>(a, ai, ov[u], oi[u]));
+                    if (better) { a = ov[u]; ai = oi[u]; }
+                }
             }
         }
 #pragma unroll
@@ -469,7 +483,7 @@ __device__ __forceinline__ void set_cand(P pts, uint64_t sbeg, uint64_t send, ui
     // the other points, which is what the reductions above computed.
     {
         float pr0[7];
-        slab7(pts.load(sbeg), pr0);
+        slab7(pt0, pr0);
 #pragma unroll
         for (int k = 0; k < 7; ++k) {
             if (pr0[k] != pr0[k]) {

@@ -240,8 +240,13 @@ def random_tree(n_leaves: int, seed: int):
     return nodes, parents
 
 
-@pytest.mark.parametrize("n", [5000, 20000])
+@pytest.mark.parametrize("n", [5000, 20000, 100000])
 def test_obb_tree_random_topology(torch, oracle_lib, n):
+    """n = 100 000 spans four 32768-leaf chunks: nearly every node has leaves
+    in several workgroups, so the climb defers most parents across workgroups,
+    folds split pairs into the global arrival words, overflows the last
+    phase's LDS slots into global arrivals, and the refit's tiles have
+    gapped ancestors everywhere."""
     say(f"[obb_tree] random topology n={n}")
     V, F = synth.hairball(tubes=n // 40 + 1, seed=6)
     F = F[:n]
