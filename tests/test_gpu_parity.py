@@ -286,3 +286,55 @@ def test_obb_tree_contiguous_topology(torch, oracle_lib, n, spine):
     assert np.array_equal(ids.cpu().numpy().view(np.uint32), ei)
     assert np.array_equal(obbs.cpu().numpy(), eo)
     assert np.array_equal(mats.cpu().numpy(), em)
+
+
+def renumber(nodes, parents, perm, lperm):
+    """The same tree with internal node i stored at perm[i] (perm[0] = 0) and
+    the leaf at position j stored at position lperm[j]."""
+    n = len(lperm)
+    I = n - 1
+    old2new = np.concatenate([perm, I + lperm]).astype(np.int64)
+    new_nodes = np.zeros_like(nodes)
+    new_parents = np.zeros_like(parents)
+    new_nodes[old2new] = nodes
+    for i in range(I):
+        j = old2new[i]
+        new_nodes[j]["a"] = old2new[nodes[i]["a"]]
+        new_nodes[j]["b"] = old2new[nodes[i]["b"]]
+    p = parents.astype(np.int64)
+    new_parents[old2new] = np.where(p == 0xFFFFFFFF, 0xFFFFFFFF, old2new[np.minimum(p, 2 * n - 2)]).astype(np.uint32)
+    return new_nodes, new_parents, old2new
+
+
+@pytest.mark.parametrize("mesh,builder", [("ico", "lbvh"), ("hair", "atrbvh")])
+def test_obb_tree_renumbering_canary(torch, oracle_lib, mesh, builder):
+    """Determinism canary: the same tree stored with its internal nodes in a
+    random order and its leaf slots reversed.  Every tile, chunk, workgroup
+    and lane assignment of the climb, the candidate search and the refit
+    changes (and the spans get gaps everywhere), so any result that
+    depended on scheduling would move; every box and matrix, mapped back,
+    must be bit-identical to the original tree's."""
+    say(f"[obb_tree] renumbering canary {mesh} {builder}")
+    V, F = synth.icosphere(48, seed=11) if mesh == "ico" else synth.hairball(tubes=1500, seed=12)
+    if builder == "lbvh":
+        nodes, parents, *_ = oracle_lib.lbvh_build(V, F)
+    else:
+        nodes, parents, *_ = oracle_lib.atrbvh_build(V, F, 7, 2)
+    n = F.shape[0]
+    rng = np.random.default_rng(n)
+    perm = np.concatenate([[0], 1 + rng.permutation(n - 2)])
+    lperm = np.arange(n)[::-1].copy()
+    nn, npar, old2new = renumber(nodes, parents, perm, lperm)
+    Vd = torch.from_numpy(V).cuda()
+    Fd = torch.from_numpy(np.ascontiguousarray(F).view(np.int32)).cuda()
+    outs = []
+    for nd_, pr_ in ((nodes, parents), (nn, npar)):
+        o, m = ops.obb_tree(torch.from_numpy(np.ascontiguousarray(nd_).view(np.uint8)).cuda(),
+                            torch.from_numpy(np.ascontiguousarray(pr_).view(np.int32)).cuda(), Vd, Fd)
+        torch.cuda.synchronize()
+        outs.append((o.cpu().numpy(), m.cpu().numpy()))
+    (o0, m0), (o1, m1) = outs
+    assert np.array_equal(o1[old2new], o0)
+    assert np.array_equal(m1[old2new], m0)
+    eo, em = oracle_lib.obb_tree(nodes, parents, V, F)
+    assert np.array_equal(o0, eo) and np.array_equal(m0, em)
