@@ -719,6 +719,19 @@ def main() -> int:
         roofline_c3["workload"] = (f"ATRBVH (treelet 7, 2 passes) -> OBB-tree transform, hairball of 50000 tubes: "
                                    f"{c3['triangles']} triangles, {c3['internal_nodes']} internal nodes")
         roofline_c3["kernel"] = CALL_KERNELS
+    # The whole call against the VALU issue peak: the profiled VALU wave
+    # instructions of one call over this run's device time per call.
+    issue_peak = 1024 * 2.4e9 / 4
+    valu_call = {}
+    for key, prof, ms in (("c2", "transform_icosphere224_valu.txt", dev_ms_mean),
+                          ("c3", "transform_hairball2m_valu.txt",
+                           (c3 or {}).get("roofline", {}).get("device_ms"))):
+        vw = pmc_value(prof, "call_valu_wave_instructions")
+        if vw and ms and (key != "c2" or args.freq == 224):
+            valu_call[key] = {"bound": "valu_issue", "achieved": round(vw / (ms * 1e-3) / 1e9, 1),
+                              "peak": round(issue_peak / 1e9, 1), "unit": "G wave-instructions/s",
+                              "frac": round(vw / (ms * 1e-3) / issue_peak, 4), "wave_instructions": int(vw),
+                              "source": "profiles/" + prof}
     valu = None
     vi, vus = (pmc_value("transform_icosphere224_valu.txt", "candidates_valu_wave_instructions_per_call"),
                pmc_value("transform_icosphere224_valu.txt", "candidates_avg_us"))
@@ -763,6 +776,7 @@ def main() -> int:
         },
         "roofline_c3": roofline_c3,
         "valu_roofline_candidates": valu,
+        "valu_roofline_call": valu_call or None,
         "cpu_baseline": cpu,
         "cpu_reference_transform": main_leg.get("cpu_reference_transform"),
         "host": host_info(),
