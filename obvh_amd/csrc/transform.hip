@@ -638,6 +638,7 @@ __global__ __launch_bounds__(kTile) void climb_leaves(const kr_bvh_node* __restr
 constexpr int kPSlots = 600;   // LDS hash slots (both children's k-DOPs each)
 constexpr int kPProbe = 64;
 constexpr int kPQueue = 1024;  // entries taken from the previous phase's list
+constexpr int kPhaseThreads = 512;  // workgroup size of climb_phase (the pipeline needs ~150 VGPRs)
 
 struct PhaseTile {
     float4 kd[kPSlots][2][7];  // side 0: left child, side 1: right child
@@ -661,6 +662,41 @@ __device__ __forceinline__ uint32_t phase_slot(PhaseTile& P, uint32_t p) {
         h = (h + 1 == (uint32_t)kPSlots) ? 0u : h + 1;
     }
     return kInvalid;
+}
+
+// Children, parent and arrival word of node p (zero / kInvalid past the
+// internal nodes).
+struct Topo {
+    uint32_t l, r, parent;
+    uint64_t v;
+};
+__device__ __forceinline__ Topo topo_load(const kr_bvh_node* __restrict__ nodes, const uint32_t* __restrict__ parents,
+                                          const TreeWs& ws, uint32_t p, uint32_t I) {
+    Topo T = {0u, 0u, kInvalid, 0ull};
+    if (p < I) {
+        T.l = nodes[p].u.children.left;
+        T.r = nodes[p].u.children.right;
+        T.parent = parents[p];
+        T.v = ws.visit[p];
+    }
+    return T;
+}
+// The k-DOP, span and leaf count of the other child of the node whose
+// topology is T, read only when T's arrival word says that child finished
+// in an earlier kernel (visible since the launch).
+struct Sib {
+    Kdop k;
+    uint2 s;
+    uint32_t c;
+};
+__device__ __forceinline__ void sib_load(const TreeWs& ws, const Topo& T, uint32_t child, uint32_t phase,
+                                         uint32_t I, uint32_t total, Sib& S) {
+    const uint32_t c0 = (uint32_t)T.v, t0 = (uint32_t)(T.v >> 32);
+    const uint32_t sib = T.l == child ? T.r : T.l;
+    if (c0 == 1u && (t0 >> 24) < phase && sib < total) {
+        kdop_load(ws.kdop, sib, S.k);
+        S.s = span_load(ws, sib, I, S.c);
+    }
 }
 
 // One phase as workgroup `wg` of `nwg`.
@@ -729,27 +765,31 @@ __device__ __forceinline__ void phase_pass(PhaseTile& P, const kr_bvh_node* __re
         kdop_store(ws.kdop, cur, k);
         span_store(ws, cur, span, count);
         completed(cur);
-        // the parent's topology and arrival word are read a level ahead
+        // Pipeline: the topology and arrival word of a node are read two
+        // levels ahead (A: the parent p, B: its parent), and where the
+        // sibling at the next level finished in an earlier kernel its k-DOP,
+        // span and count are read as soon as this level's merge is done
+        // (SA), so that a level costs about one memory round trip.
         uint32_t p = parents[cur];
-        uint32_t l = 0, rr = 0, pp = kInvalid;
-        uint64_t v = 0;
-        if (p < I) { l = nodes[p].u.children.left; rr = nodes[p].u.children.right; pp = parents[p]; v = ws.visit[p]; }
+        Topo A = topo_load(nodes, parents, ws, p, I);
+        Sib SA;
+        sib_load(ws, A, cur, phase, I, total, SA);
+        Topo B = topo_load(nodes, parents, ws, A.parent, I);
         for (uint32_t step = 0; p != kInvalid; ++step) {
             if (p >= I || step >= N) { atomicOr(ws.error, 1u); break; }
-            const bool am_left = (l == cur);
-            const uint32_t sib = am_left ? rr : l;
-            if (sib >= total || (!am_left && rr != cur)) { atomicOr(ws.error, 2u); break; }
-            uint32_t nl = 0, nr = 0, npp = kInvalid;
-            uint64_t nv = 0;
-            if (pp < I) { nl = nodes[pp].u.children.left; nr = nodes[pp].u.children.right; npp = parents[pp]; nv = ws.visit[pp]; }
-            const uint32_t c0 = (uint32_t)v, t0 = (uint32_t)(v >> 32);
+            const bool am_left = (A.l == cur);
+            const uint32_t sib = am_left ? A.r : A.l;
+            if (sib >= total || (!am_left && A.r != cur)) { atomicOr(ws.error, 2u); break; }
+            const Topo C = topo_load(nodes, parents, ws, B.parent, I);
+            const uint32_t c0 = (uint32_t)A.v, t0 = (uint32_t)(A.v >> 32);
             Kdop o;
             uint2 so;
             uint32_t co;
             if (c0 == 1u && (t0 >> 24) < phase) {  // the sibling finished in an earlier kernel
-                kdop_load(ws.kdop, sib, o);
-                so = span_load(ws, sib, I, co);
-                ws.visit[p] = v + (((uint64_t)tag << 32) | 1ull);
+                o = SA.k;
+                so = SA.s;
+                co = SA.c;
+                ws.visit[p] = A.v + (((uint64_t)tag << 32) | 1ull);
             } else if (c0 > 1u) {
                 atomicOr(ws.error, 2u);
                 break;
@@ -793,12 +833,11 @@ __device__ __forceinline__ void phase_pass(PhaseTile& P, const kr_bvh_node* __re
             kdop_store(ws.kdop, p, k);
             span_store(ws, p, span, count);
             completed(p);
+            sib_load(ws, B, p, phase, I, total, SA);  // the next level's sibling
             cur = p;
-            p = pp;
-            l = nl;
-            rr = nr;
-            pp = npp;
-            v = nv;
+            p = A.parent;
+            A = B;
+            B = C;
         }
     }
     __syncthreads();
@@ -825,7 +864,7 @@ __device__ __forceinline__ void phase_pass(PhaseTile& P, const kr_bvh_node* __re
 // last phase over the whole tree on the CU it holds, so that the last phase
 // does not wait for a CU to drain while the side stream fills the chip.  A
 // launch with one workgroup runs its phase as the last.
-__global__ __launch_bounds__(1024) void climb_phase(const kr_bvh_node* __restrict__ nodes,
+__global__ __launch_bounds__(kPhaseThreads) void climb_phase(const kr_bvh_node* __restrict__ nodes,
                                                     const uint32_t* __restrict__ parents, TreeWs ws, uint32_t N,
                                                     uint32_t phase, uint32_t chunk) {
     __shared__ PhaseTile P;
@@ -1888,7 +1927,7 @@ kr_error tree_launch(const kr_bvh_node* nodes, const uint32_t* parents, const kr
     }
     if (N > kTile) {  // phase 1 (and its last workgroup phase 2), or one phase for a small tree
         const uint32_t c = N > kChunk1 ? kChunk1 : N;
-        hipLaunchKernelGGL(climb_phase, blocks(N, c), dim3(1024), 0, stream, nodes, parents, ws, N, 1u, c);
+        hipLaunchKernelGGL(climb_phase, blocks(N, c), dim3(kPhaseThreads), 0, stream, nodes, parents, ws, N, 1u, c);
     }
     if (I > 0) {
         // the nodes the phases completed, and the refit tiles' chains
