@@ -1,7 +1,8 @@
 """obvh_obb_tree_batch: many trees in one transform.  Every tree's boxes and
 matrices must equal the single-tree call's (obvh_obb_tree_async) and the
 oracle's, bit for bit, for batches of uneven LBVH and ATRBVH trees
-(including one- and two-triangle trees); the call replays from a HIP graph;
+(including one- and two-triangle trees) and of random topologies (every
+span gapped); the call replays from a HIP graph;
 a malformed tree is reported through the batch status word."""
 import sys
 
@@ -12,6 +13,7 @@ pytestmark = pytest.mark.gpu
 
 from obvh_amd import ops, synth  # noqa: E402
 from obvh_amd.layouts import NODE_DTYPE  # noqa: E402
+from treeutil import random_tree  # noqa: E402
 
 
 @pytest.fixture(scope="module")
@@ -43,9 +45,11 @@ def meshes(seed):
 
 def build_trees(oracle_lib, ms, builder):
     trees = []
-    for V, F in ms:
+    for k, (V, F) in enumerate(ms):
         if builder == "lbvh" or F.shape[0] < 2:
             nodes, parents, *_ = oracle_lib.lbvh_build(V, F)
+        elif builder == "random":  # shuffled leaves, breadth-first internal nodes
+            nodes, parents = random_tree(F.shape[0], seed=1000 + k)
         else:
             nodes, parents, *_ = oracle_lib.atrbvh_build(V, F, 7, 2)
         trees.append((nodes, parents, V, F))
@@ -62,9 +66,9 @@ def to_dev(torch, trees):
     return dev
 
 
-@pytest.mark.parametrize("builder", ["lbvh", "atrbvh"])
+@pytest.mark.parametrize("builder", ["lbvh", "atrbvh", "random"])
 def test_batch_equals_single_tree_calls_and_oracle(torch, oracle_lib, builder):
-    ms = meshes(7 if builder == "lbvh" else 8)
+    ms = meshes({"lbvh": 7, "atrbvh": 8, "random": 13}[builder])
     say(f"[tree batch] {builder}: {len(ms)} trees, {sum(F.shape[0] for _, F in ms)} triangles")
     trees = build_trees(oracle_lib, ms, builder)
     dev = to_dev(torch, trees)
