@@ -462,7 +462,7 @@ __global__ __launch_bounds__(kTile) void climb_leaves(const kr_bvh_node* __restr
     const uint32_t pos = lo + t;
 
     // tile topology and the lane's leaf
-    Kdop k;
+    Kdop k = {};
     bool climbing = false;
     f3 v0 = f3{0, 0, 0}, v1 = v0, v2 = v0;  // the lane's triangle, for its leaf box
     if (pos < N) {
@@ -496,26 +496,82 @@ __global__ __launch_bounds__(kTile) void climb_leaves(const kr_bvh_node* __restr
     L.arrive[t] = 0;
     __syncthreads();
 
-    if (climbing) {
-        uint32_t cur = I + pos;
-        uint2 span = leaf_span(pos);
-        uint32_t count = 1;
-        uint32_t p = L.par_leaf[t];
+    // First level, all lanes together: two leaf siblings in one wave are
+    // merged by the left one (the right one's k-DOP comes by a lane shuffle)
+    // without an arrival count.  Any other leaf with a parent in the tile
+    // leaves its k-DOP in the parent's LDS slot before it arrives (when its
+    // sibling is internal, or it is the left one of two leaves), so that a
+    // sibling arriving second reads it from LDS instead of recomputing it.
+    uint32_t cur = I + pos;
+    uint2 span = leaf_span(pos);
+    uint32_t count = 1;
+    uint32_t p = L.par_leaf[t];
+    bool climb = climbing;  // the lane climbs on past the first level
+    {
+        const uint64_t climbers = __ballot(climbing);
+        uint32_t src = (uint32_t)lane_id(), ps = p - lo, sib = kInvalid;
+        bool pair_left = false, pair_right = false, stash = false;
+        if (climbing && p < I && ps < kTile) {
+            const uint2 kc = L.kids[ps];
+            const bool am_left = kc.x == cur;
+            if (am_left || kc.y == cur) {  // otherwise the climb below reports it
+                sib = am_left ? kc.y : kc.x;
+                const uint32_t sl = sib - I - (pos - (uint32_t)lane_id());  // the sibling leaf's lane
+                if (sib >= I && sib < total && sl < (uint32_t)kWave && ((climbers >> sl) & 1ull)) {
+                    pair_left = am_left;
+                    pair_right = !am_left;
+                    src = sl;
+                } else {
+                    stash = sib < I || am_left;
+                }
+            }
+        }
+        Kdop o;
+#pragma unroll
+        for (int q = 0; q < 7; ++q) {  // every lane of the wave takes part
+            o.mn[q] = __shfl(k.mn[q], (int)src, kWave);
+            o.mx[q] = __shfl(k.mx[q], (int)src, kWave);
+            o.imn[q] = (uint32_t)__shfl((int)k.imn[q], (int)src, kWave);
+            o.imx[q] = (uint32_t)__shfl((int)k.imx[q], (int)src, kWave);
+        }
+        if (pair_right) climb = false;  // the left sibling merges
+        if (pair_left) {
+            Kdop m;
+            merge_left_first(k, o, m);
+            k = m;
+            span = span_join(span, count, leaf_span(sib - I), 1u, count);
+#pragma unroll
+            for (int q = 0; q < 7; ++q)
+                L.kd[ps][q] = make_float4(k.mn[q], __uint_as_float(k.imn[q]), k.mx[q], __uint_as_float(k.imx[q]));
+            L.span[ps] = span;
+            L.cnt[ps] = count;
+            L.arrive[ps] = 2u;  // both children: no other lane arrives here
+            cur = p;
+            p = L.par_int[ps];
+        }
+        if (stash) {
+#pragma unroll
+            for (int q = 0; q < 7; ++q)
+                L.kd[ps][q] = make_float4(k.mn[q], __uint_as_float(k.imn[q]), k.mx[q], __uint_as_float(k.imx[q]));
+        }
+    }
+
+    if (climb) {
         for (uint32_t step = 0; p != kInvalid; ++step) {
             if (p >= I || step >= N) { atomicOr(ws.error, 1u); break; }
             const uint32_t s = p - lo;
             uint32_t sib, pp;
             bool am_left;
             if (s < kTile) {  // parent in the tile: LDS arrival
+                const uint2 kc = L.kids[s];  // topology: read before the arrival
+                pp = L.par_int[s];
                 const uint32_t cnt =
                     __hip_atomic_fetch_add(&L.arrive[s], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
                 if (cnt == 0) break;  // first arrival: the sibling continues
                 if (cnt != 1) { atomicOr(ws.error, 2u); break; }
-                const uint2 kc = L.kids[s];
                 am_left = kc.x == cur;
                 if (!am_left && kc.y != cur) { atomicOr(ws.error, 2u); break; }
                 sib = am_left ? kc.y : kc.x;
-                pp = L.par_int[s];
             } else {  // parent outside the tile: publish, then global arrival
                 kdop_store(ws.kdop, cur, k);
                 span_store(ws, cur, span, count);
@@ -538,7 +594,19 @@ __global__ __launch_bounds__(kTile) void climb_leaves(const kr_bvh_node* __restr
             if (sib >= I) {
                 const uint32_t ss = sib - I - lo;
                 if (sib >= total || ss >= kTile) { atomicOr(ws.error, 2u); break; }
-                tile_leaf_kdop(L, V, ss, o);
+                // a leaf left its k-DOP in the parent's slot unless this lane
+                // is its left leaf sibling (from another wave)
+                const uint32_t from = s < kTile && (cur < I || !am_left) ? s : kInvalid;
+                if (from != kInvalid) {
+#pragma unroll
+                    for (int q = 0; q < 7; ++q) {
+                        const float4 a = L.kd[from][q];
+                        o.mn[q] = a.x; o.imn[q] = __float_as_uint(a.y);
+                        o.mx[q] = a.z; o.imx[q] = __float_as_uint(a.w);
+                    }
+                } else {
+                    tile_leaf_kdop(L, V, ss, o);
+                }
                 so = leaf_span(sib - I);
                 co = 1;
             } else if (sib - lo < kTile) {  // completed in this tile: k-DOP in LDS
