@@ -416,9 +416,11 @@ __device__ __forceinline__ bool leaf_kdop_global(const kr_bvh_node* nodes, const
 // and checks that every child of its internal nodes points back to it.
 // Then one lane per leaf climbs: at a parent in the tile the arrival is
 // counted in LDS, the first child stops and the second merges, reading the
-// sibling's k-DOP from LDS (internal node of the tile), recomputing it
-// (leaf of the tile) or loading it (node completed outside the tile's
-// index range).  Parents outside the tile use the global arrival word with
+// sibling's k-DOP from LDS (internal node of the tile, or a leaf's k-DOP
+// left in the parent's slot), recomputing it (a leaf that left none) or
+// loading it (node completed outside the tile's index range).  At the
+// first level two leaf siblings in one wave merge without an arrival count:
+// the left one takes the right one's k-DOP by lane shuffles.  Parents outside the tile use the global arrival word with
 // the workgroup's tag; such a parent is deferred to the next phase when its
 // children finished in different workgroups.  Finally tile nodes reached by
 // one child only are folded into the global arrival words, and the k-DOPs,
