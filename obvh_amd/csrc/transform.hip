@@ -212,6 +212,22 @@ __device__ __forceinline__ void stage_nan(StageT<R>& st, uint32_t r) {
     for (int q = 0; q < 16; ++q) st.mt[r * 17 + q] = nan;
 }
 
+// Dwords g[0 .. n) = at(j), written by a workgroup of R threads as 16-byte
+// stores from the first 16-byte boundary of g on, the dwords before it and
+// after the last full vector one by one.
+template <int R, class At>
+__device__ __forceinline__ void flush_dwords(float* g, uint32_t n, At at) {
+    const uint32_t h = min((uint32_t)(((16u - ((uint32_t)(uintptr_t)g & 15u)) & 15u) >> 2), n);
+    const uint32_t body = (n - h) >> 2;
+    for (uint32_t j = threadIdx.x; j < h; j += R) g[j] = at(j);
+    float4* gv = reinterpret_cast<float4*>(g + h);
+    for (uint32_t v = threadIdx.x; v < body; v += R) {
+        const uint32_t j = h + 4 * v;
+        gv[v] = make_float4(at(j), at(j + 1), at(j + 2), at(j + 3));
+    }
+    for (uint32_t j = h + 4 * body + threadIdx.x; j < n; j += R) g[j] = at(j);
+}
+
 // Rows first .. first + cnt - 1 (combined indices) to their output nodes: the
 // same node indices, or for a batch each tree's own layout (runs of
 // consecutive rows stay consecutive; top nodes are dropped).
@@ -219,10 +235,9 @@ template <int R>
 __device__ __forceinline__ void stage_flush(StageT<R>& st, kr_obb3* obbs, kr_mat4* mats, size_t first, uint32_t cnt,
                                             const BatchMap& m) {
     if (m.K == 0) {
-        float* go = reinterpret_cast<float*>(obbs) + first * 15;
-        for (uint32_t j = threadIdx.x; j < cnt * 15; j += R) go[j] = st.ob[j];
-        float* gm = reinterpret_cast<float*>(mats) + first * 16;
-        for (uint32_t j = threadIdx.x; j < cnt * 16; j += R) gm[j] = st.mt[(j >> 4) * 17 + (j & 15)];
+        flush_dwords<R>(reinterpret_cast<float*>(obbs) + first * 15, cnt * 15, [&](uint32_t j) { return st.ob[j]; });
+        flush_dwords<R>(reinterpret_cast<float*>(mats) + first * 16, cnt * 16,
+                        [&](uint32_t j) { return st.mt[(j >> 4) * 17 + (j & 15)]; });
         return;
     }
     if (threadIdx.x < cnt) st.orow[threadIdx.x] = batch_out_index(m, (uint32_t)first + threadIdx.x);
@@ -1150,8 +1165,24 @@ __device__ __forceinline__ void cand_search(const Pts<NP, kRef>& L, f3 p0, f3 p1
             nmx = fmaxf(nmx, pn[i]);
         }
     }
-    face(L, nn, nmn, nmx, e0, e1, e2, pe0, pe1, pe2, b);
+    // the apexes' validity first: a node without one is degenerate and
+    // keeps b at the AABB (its range then holds the AABB for finalize)
     float lo = pn[0], hi = pn[0];
+#pragma unroll
+    for (int i = 1; i < NP; ++i) {
+        lo = pn[i] < lo ? pn[i] : lo;
+        hi = pn[i] > hi ? pn[i] : hi;
+    }
+    const float tri = dot3(p0, nn);
+    const bool q0v = hi - eps > tri;
+    const bool q1v = lo + eps < tri;
+    if (!q0v && !q1v) {
+        degen = 1;
+        return;
+    }
+    face(L, nn, nmn, nmx, e0, e1, e2, pe0, pe1, pe2, b);
+    lo = pn[0];
+    hi = pn[0];
     f3 q1 = P[0], q0 = P[0];
 #pragma unroll
     for (int i = 1; i < NP; ++i) {
@@ -1160,13 +1191,6 @@ __device__ __forceinline__ void cand_search(const Pts<NP, kRef>& L, f3 p0, f3 p1
         q1 = pick(cl, P[i], q1);
         hi = ch ? pn[i] : hi;
         q0 = pick(ch, P[i], q0);
-    }
-    const float tri = dot3(p0, nn);
-    const bool q0v = hi - eps > tri;
-    const bool q1v = lo + eps < tri;
-    if (!q0v && !q1v) {
-        degen = 1;
-        return;
     }
     // apex a = 0: q0 (candidates 3..11), a = 1: q1 (12..20); on apex q the
     // faces j = 0, 1, 2 have edges (e_j, f_{j+1}, f_j) with f_i = q - p_i
@@ -1233,7 +1257,10 @@ __device__ __forceinline__ bool sane_points(const f3* P, int n) {
 }
 
 // aq: half area of the node's AABB (the k-DOP's first three slabs), kept
-// so that finalize reads the k-DOP only where the AABB wins.
+// so that finalize reads the k-DOP only where the AABB wins.  The range
+// starts as the result's projection range: for a degenerate node (which
+// the refit skips and finalize boxes by its AABB) the search leaves b at
+// the AABB, so finalize reads that from the range, not from the k-DOP.
 __device__ __forceinline__ void store_cand(TreeWs& ws, uint32_t node, const Best& b, int degen, float aq) {
     Cand c;
     c.b[0] = b.b0.x; c.b[1] = b.b0.y; c.b[2] = b.b0.z;
@@ -1867,13 +1894,18 @@ __global__ __launch_bounds__(kStage) void finalize(TreeWs ws, kr_obb3* obbs, kr_
                 mn = rmn;
                 mx = rmx;
             } else {  // the AABB: the k-DOP's first three slabs
-                const float4* kd = &ws.kdop[(size_t)node * 7];
-                const float4 s0 = kd[0], s1 = kd[1], s2 = kd[2];
                 v0 = f3{1, 0, 0};
                 v1 = f3{0, 1, 0};
                 v2 = f3{0, 0, 1};
-                mn = f3{s0.x, s1.x, s2.x};
-                mx = f3{s0.z, s1.z, s2.z};
+                if (c.degen) {  // the candidate step kept them in the range, which the refit skips
+                    mn = rmn;
+                    mx = rmx;
+                } else {
+                    const float4* kd = &ws.kdop[(size_t)node * 7];
+                    const float4 s0 = kd[0], s1 = kd[1], s2 = kd[2];
+                    mn = f3{s0.x, s1.x, s2.x};
+                    mx = f3{s0.z, s1.z, s2.z};
+                }
             }
             const f3 mid = vm::mul1(add3(mn, mx), 0.5f);
             const f3 ext = vm::vmax3(vm::mul1(sub3(mx, mn), 0.5f), f3{0.001f, 0.001f, 0.001f});
