@@ -442,9 +442,9 @@ __device__ __forceinline__ void set_cand(P pts, uint64_t sbeg, uint64_t send, ui
         float a = mn ? FLT_MAX * 2.0f : -FLT_MAX * 2.0f;
         uint32_t ai = kNoIdx;
         if (side < 14) {
-            // eight tiles' pairs are read before any is compared: one memory
-            // round trip per 32 tiles instead of one per 4
-            constexpr int kB = 8;
+            // sixteen tiles' pairs are read before any is compared: one
+            // memory round trip per 64 tiles instead of one per 4
+            constexpr int kB = 16;
             for (uint32_t j0 = (uint32_t)(lane >> 4); j0 < nt; j0 += 4 * kB) {
                 float ov[kB];
                 uint32_t oi[kB];
