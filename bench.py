@@ -177,27 +177,45 @@ def dito_legs(args) -> dict:
     out["batched_over_plain"] = round((args.sets / dev) / (args.sets / plain), 2)
     print(f"[bench] dito plain per set: {plain*1e3:.1f} ms ({out['batched_over_plain']}x slower), equal={same}",
           file=sys.stderr, flush=True)
-    # one 131 072-point set
+    # one 131 072-point set: back-to-back calls of the C entry points (ctypes,
+    # no allocation per call), so that the host does not set the pace
+    import ctypes
+    from obvh_amd import _lib
     p = synth.gaussian_points(131072, seed=1)
     xs, ys, zs = (torch.from_numpy(np.ascontiguousarray(p[:, k])).cuda() for k in range(3))
     ws1 = torch.empty(ops.lib().obvh_dito_workspace_size(131072), dtype=torch.uint8, device="cuda")
     o1 = torch.empty(15, dtype=torch.float32, device="cuda")
-    for _ in range(5):
-        ops.dito(xs, ys, zs, workspace=ws1)
-    torch.cuda.synchronize()
-    e0.record()
-    for _ in range(50):
-        ops.dito(xs, ys, zs, workspace=ws1)
-    e1.record()
-    torch.cuda.synchronize()
-    single = e0.elapsed_time(e1) / 50 / 1e3
+    L = _lib.lib()
+    cs = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    vp = lambda t: ctypes.c_void_p(t.data_ptr())
+    fits = {
+        "soa": lambda: _lib.check(L.obvh_dito_soa(vp(xs), vp(ys), vp(zs), 131072, vp(o1), None, vp(ws1),
+                                                  ws1.numel(), cs)),
+        "noalloc": lambda: _lib.check(L.obvh_dito_soa_noalloc(vp(xs), vp(ys), vp(zs), 131072, vp(o1), None, cs)),
+    }
+    fit_us = {}
+    for name, fit in fits.items():
+        for _ in range(10):
+            fit()
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(200):
+            fit()
+        e1.record()
+        torch.cuda.synchronize()
+        fit_us[name] = e0.elapsed_time(e1) / 200 * 1e3
+    single = fit_us["noalloc"] / 1e6
     legacy = []
     for _ in range(5):
         w0 = time.perf_counter()
         r = ops.points_obb(p)
         legacy.append(time.perf_counter() - w0)
     out["single_131072"] = {"device_us": round(single * 1e6, 2), "Gpts_per_s": round(131072 / single / 1e9, 3),
-                            "legacy_wall_us": round(min(legacy) * 1e6, 1)}
+                            "entry": "obvh_dito_soa_noalloc (two launches)",
+                            "soa_device_us": round(fit_us["soa"], 2),
+                            "legacy_wall_us": round(min(legacy) * 1e6, 1),
+                            "equal_to_legacy": bool(np.array_equal(o1.cpu().numpy(),
+                                                                   np.asarray(r, dtype=np.float32).reshape(-1)))}
     print(f"[bench] dito single 131072: {single*1e6:.1f} us device, legacy {min(legacy)*1e6:.0f} us wall",
           file=sys.stderr, flush=True)
     del o1, r
