@@ -1,7 +1,8 @@
 """Pins of the CPU DiTO-14 oracle (oracle/dito.c) by closed forms, brute force
-and invariants.  The reference CPU sources cannot be compiled here (see
-DESIGN.md), so nothing here compares against the reference binary; each test
-states what fixes the expected answer independently of the oracle's code.
+and invariants.  The reference CPU sources cannot be compiled here (its
+common/vecmath.h includes the MSVC header <intrin.h>), so nothing here
+compares against the reference binary; each test states what fixes the
+expected answer independently of the oracle's code.
 """
 import numpy as np
 import pytest
