@@ -39,11 +39,22 @@ def mesh(kind):
         return synth.hairball(tubes=1, seed=5)
     if kind == "hair200k":
         return synth.hairball(tubes=5000, seed=6)
+    if kind == "flat":  # a jittered grid in the plane z = 0.5: zero-thickness boxes, many tied costs
+        n = 50
+        rng = np.random.default_rng(7)
+        g = np.stack(np.meshgrid(np.arange(n), np.arange(n), indexing="ij"), -1).reshape(-1, 2) + rng.uniform(
+            -0.3, 0.3, (n * n, 2))
+        V = np.concatenate([g / n, np.full((n * n, 1), 0.5)], 1).astype(np.float32)
+        i, j = np.meshgrid(np.arange(n - 1), np.arange(n - 1), indexing="ij")
+        a, b = (i * n + j).ravel(), (i * n + j + 1).ravel()
+        c, d = ((i + 1) * n + j).ravel(), ((i + 1) * n + j + 1).ravel()
+        F = np.concatenate([np.stack([a, b, c], 1), np.stack([b, d, c], 1)])
+        return V, np.concatenate([F, np.zeros((len(F), 1), np.int64)], 1).astype(np.uint32)
     raise ValueError(kind)
 
 
 @pytest.mark.parametrize("kind,ts,its", [("ico", 7, 2), ("hair", 7, 2), ("hair", 9, 2), ("dup", 7, 2),
-                                         ("tiny", 7, 2), ("hair", 7, 0), ("hair200k", 7, 2)])
+                                         ("tiny", 7, 2), ("hair", 7, 0), ("hair200k", 7, 2), ("flat", 7, 2)])
 def test_atrbvh_matches_oracle(torch, oracle_lib, kind, ts, its):
     say(f"[atrbvh] {kind} treelet={ts} passes={its}")
     V, F = mesh(kind)
@@ -58,9 +69,10 @@ def test_atrbvh_matches_oracle(torch, oracle_lib, kind, ts, its):
     assert np.array_equal(t["costs"].cpu().numpy(), eco)
 
 
-def test_atrbvh_then_transform_and_collapse(torch, oracle_lib):
-    say("[atrbvh] transform + collapse on the optimised tree")
-    V, F = mesh("hair")
+@pytest.mark.parametrize("kind", ["hair", "flat"])
+def test_atrbvh_then_transform_and_collapse(torch, oracle_lib, kind):
+    say(f"[atrbvh] transform + collapse on the optimised tree ({kind})")
+    V, F = mesh(kind)
     Vd = torch.from_numpy(V).cuda()
     Fd = torch.from_numpy(np.ascontiguousarray(F).view(np.int32)).cuda()
     t = ops.atrbvh_build(Vd, Fd, 7, 2, CI, CT)
