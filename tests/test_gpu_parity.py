@@ -58,6 +58,9 @@ DITO_CASES = {
     "dups": lambda: (np.random.default_rng(6).integers(-8, 8, (5000, 3)) * 0.25).astype(np.float32),
     "line": lambda: (np.linspace(-1, 1, 777)[:, None] * np.array([0.3, -0.5, 0.8])).astype(np.float32),
     "point": lambda: np.tile(np.array([[0.5, 0.5, 0.5]], np.float32), (600, 1)),
+    # coplanar points (a tilted plane through the origin, exact in float): no apex off the base triangle
+    "plane": lambda: (np.random.default_rng(8).integers(-64, 64, (3000, 2)) @ np.array(
+        [[0.5, 0.25, 0.0], [0.0, 0.5, 0.25]])).astype(np.float32),
     "ragged": lambda: synth.gaussian_points(512 * 7 + 13, seed=7),
 }
 
