@@ -1,8 +1,9 @@
 """obvh_obb_tree_batch: many trees in one transform.  Every tree's boxes and
 matrices must equal the single-tree call's (obvh_obb_tree_async) and the
 oracle's, bit for bit, for batches of uneven LBVH and ATRBVH trees
-(including one- and two-triangle trees) and of random topologies (every
-span gapped); the call replays from a HIP graph;
+(including one- and two-triangle trees), of random topologies (every span
+gapped) and of degenerate meshes (flat grids, doubled vertices); the call
+replays from a HIP graph;
 a malformed tree is reported through the batch status word."""
 import sys
 
@@ -13,7 +14,7 @@ pytestmark = pytest.mark.gpu
 
 from obvh_amd import ops, synth  # noqa: E402
 from obvh_amd.layouts import NODE_DTYPE  # noqa: E402
-from treeutil import random_tree  # noqa: E402
+from treeutil import doubled_vertices, flat_grid, random_tree  # noqa: E402
 
 
 @pytest.fixture(scope="module")
@@ -46,7 +47,7 @@ def meshes(seed):
 def build_trees(oracle_lib, ms, builder):
     trees = []
     for k, (V, F) in enumerate(ms):
-        if builder == "lbvh" or F.shape[0] < 2:
+        if builder in ("lbvh", "degenerate") or F.shape[0] < 2:
             nodes, parents, *_ = oracle_lib.lbvh_build(V, F)
         elif builder == "random":  # shuffled leaves, breadth-first internal nodes
             nodes, parents = random_tree(F.shape[0], seed=1000 + k)
@@ -66,9 +67,18 @@ def to_dev(torch, trees):
     return dev
 
 
-@pytest.mark.parametrize("builder", ["lbvh", "atrbvh", "random"])
+def degenerate_meshes():
+    """Flat grids (every node coplanar) and icospheres with doubled vertices,
+    between one- and two-triangle meshes."""
+    ms = meshes(14)[:2]
+    for k in range(10):
+        ms.append(flat_grid(8 + 7 * k, seed=40 + k) if k % 2 == 0 else doubled_vertices(seed=40 + k))
+    return ms
+
+
+@pytest.mark.parametrize("builder", ["lbvh", "atrbvh", "random", "degenerate"])
 def test_batch_equals_single_tree_calls_and_oracle(torch, oracle_lib, builder):
-    ms = meshes({"lbvh": 7, "atrbvh": 8, "random": 13}[builder])
+    ms = degenerate_meshes() if builder == "degenerate" else meshes({"lbvh": 7, "atrbvh": 8, "random": 13}[builder])
     say(f"[tree batch] {builder}: {len(ms)} trees, {sum(F.shape[0] for _, F in ms)} triangles")
     trees = build_trees(oracle_lib, ms, builder)
     dev = to_dev(torch, trees)

@@ -1,7 +1,8 @@
 """Tree helpers shared by the tests: random topologies with AABBs, subtree
-walks.  Plain numpy, no GPU."""
+walks, degenerate meshes.  Plain numpy, no GPU."""
 import numpy as np
 
+from obvh_amd import synth
 from obvh_amd.layouts import NODE_DTYPE
 
 INV = 0xFFFFFFFF
@@ -116,3 +117,40 @@ def leaves_right_first(nodes, v, I):
     if v >= I:
         return [v]
     return leaves_right_first(nodes, int(nodes[v]["b"]), I) + leaves_right_first(nodes, int(nodes[v]["a"]), I)
+
+
+def flat_grid(n: int, seed: int):
+    """An n x n vertex grid in the plane z = 0.25 (jittered in x and y), two
+    triangles per cell: every node's points are coplanar, so the candidate
+    search finds no apex off the base plane (its third degenerate exit)."""
+    rng = np.random.default_rng(seed)
+    g = np.stack(np.meshgrid(np.arange(n), np.arange(n), indexing="ij"), -1).reshape(-1, 2).astype(np.float64)
+    g += rng.uniform(-0.3, 0.3, g.shape)
+    V = np.concatenate([g / n, np.full((n * n, 1), 0.25)], 1).astype(np.float32)
+    i, j = np.meshgrid(np.arange(n - 1), np.arange(n - 1), indexing="ij")
+    a, b, c, d = (i * n + j).ravel(), (i * n + j + 1).ravel(), ((i + 1) * n + j).ravel(), ((i + 1) * n + j + 1).ravel()
+    F = np.concatenate([np.stack([a, b, c], 1), np.stack([b, d, c], 1)])
+    return V, np.concatenate([F, np.zeros((len(F), 1), np.int64)], 1).astype(np.uint32)
+
+
+def doubled_vertices(seed: int):
+    """An icosphere whose every vertex is stored twice, the faces using either
+    copy at random: equal positions under different ids, so slab ties are
+    decided by the tie rule and distinct ids need not be distinct points."""
+    V, F = synth.icosphere(10, seed=seed)
+    rng = np.random.default_rng(seed)
+    V2 = np.concatenate([V, V])
+    F2 = F.copy()
+    F2[:, :3] += (rng.integers(0, 2, F[:, :3].shape) * len(V)).astype(np.uint32)
+    return V2, F2
+
+
+def slivers(seed: int):
+    """Hairball tubes where every third triangle is collapsed onto a segment
+    (its third vertex moved to the midpoint of the first two): zero-area
+    triangles among ordinary ones."""
+    V, F = synth.hairball(tubes=60, seed=seed)
+    V = V.copy()
+    for f in F[::3]:
+        V[f[2]] = (V[f[0]] + V[f[1]]) * np.float32(0.5)
+    return V, F
