@@ -181,3 +181,32 @@ def test_obb_tree_status_reports_malformed_tree(torch):
     word = int(host.item())
     assert word != 0
     assert "parent/child" in ops.tree_status_message(word) or "never completed" in ops.tree_status_message(word)
+
+
+@pytest.mark.parametrize("kind", ["all", "one"])
+def test_obb_tree_huge_coordinates(torch, oracle_lib, kind):
+    """Finite coordinates beyond 2^61 in magnitude: the candidate search takes
+    the reference's compare-and-select mode (the separate redo launch) for
+    every node that holds such a point, all nodes ("all": the mesh scaled by
+    2^62) or the ancestors of one far vertex ("one"), the rest their
+    distinct-point search."""
+    say(f"[obb_tree huge] kind={kind}")
+    V, F = synth.icosphere(8, seed=6)
+    Vd0 = torch.from_numpy(V).cuda()
+    Fd = torch.from_numpy(F.view(np.int32)).cuda()
+    t = ops.lbvh_build(Vd0, Fd)
+    W = V.copy()
+    if kind == "all":
+        W *= np.float32(2.0 ** 62)
+    else:
+        W[int(F[len(F) // 2, 0])] = np.array([3.0e18, -2.5e18, 1.0e18], np.float32)
+    Wd = torch.from_numpy(W).cuda()
+    obbs, mats, ids = ops.obb_tree(t["nodes"], t["parents"], Wd, Fd, with_ids=True)
+    torch.cuda.synchronize()
+    nodes = ops.nodes_to_numpy(t["nodes"])
+    parents = t["parents"].cpu().numpy().view(np.uint32)
+    eo, em, ei = oracle_lib.obb_tree(nodes, parents, W, F, with_ids=True)
+    assert np.array_equal(ids.cpu().numpy().view(np.uint32), ei)
+    go, gm = obbs.cpu().numpy(), mats.cpu().numpy()
+    bad = [v for v in range(len(go)) if not (same(go[v], eo[v]) and same(gm[v], em[v]))]
+    assert not bad, (bad[:10], go[bad[0]], eo[bad[0]])
